@@ -1,0 +1,278 @@
+#!/usr/bin/env python3
+"""WPK SpMV benchmark on one MI355X.
+
+Headline: FP64 WPK1 SpMV (engine::run, group width 64) on the 3D trilinear hex
+linear-elasticity matrix, 64^3 elements (823,875 rows, 64,701,513 nnz), one
+right-hand side, x ~ U[-1,1].  One step = one SpMV launch with the matrix and x
+already resident in HBM.
+
+    GB/s = algorithmic bytes / time,  bytes = nnz*(8+4) + ncols*8 + nrows*8
+    (padding, row_map and group metadata are overhead and are not counted)
+
+Prints ONE JSON line.  Extras (skipped with --headline-only) measure the other
+parts of the metric: run_x, WPK2, FP32-compensated multi-RHS against plain FP64
+(hex elasticity 32^3, k = 1..32) and forward + adjoint time (hex diffusion 48^3).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SpMV GB/s & % HBM roofline, FP64 vs FP32-comp multi-RHS; fwd+adjoint ms"
+
+
+def spmv_bytes(nnz, nrows, ncols, k=1, s_val=8, s_x=8, s_y=8):
+    return nnz * (s_val + 4) + k * (ncols * s_x + nrows * s_y)
+
+
+def time_launches(torch, fn, steps, warmup):
+    """Returns (wall seconds for `steps` launches, mean launch ms from HIP events
+    recorded on the launch stream)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ev0.elapsed_time(ev1) / steps
+
+
+def read_traffic(workload):
+    """HBM bytes per launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        if ln.startswith("traffic_bytes_per_launch:"):
+            return float(ln.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(m, x, budget_s):
+    """The oracle's FP64 CSR SpMV (original row order, fma), timed on this host,
+    1 thread and all cores; plus the host WPK1 scan and process."""
+    from oracle import pyoracle as O
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    b = spmv_bytes(m.nnz, m.nrows, m.ncols)
+    O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    t0 = time.perf_counter()
+    O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    one = time.perf_counter() - t0
+    reps = int(max(5, min(200, 0.4 * budget_s / max(one, 1e-6))))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+        ts.append(time.perf_counter() - t0)
+    t1 = float(np.median(ts))
+    O.csr_spmv_omp(m.rowptr, m.colind, m.vals, x, cores)
+    tsn = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.csr_spmv_omp(m.rowptr, m.colind, m.vals, x, cores)
+        tsn.append(time.perf_counter() - t0)
+    tn = float(np.median(tsn))
+    t0 = time.perf_counter()
+    L = O.scan1(m.rowptr, 64)
+    t_scan = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.process(L, m.vals, m.colind)
+    t_proc = time.perf_counter() - t0
+    return {
+        "value": round(b / tn / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+        "sample": (f"oracle FP64 CSR SpMV (original row order, fma) on the full {m.name} matrix, "
+                   f"median of {reps} reps after a warm-up, OpenMP over rows on {cores} cores; "
+                   f"1 thread: {b / t1 / 1e9:.3f} GB/s ({t1 * 1e3:.1f} ms); "
+                   f"host WPK1 scan (W=64) {t_scan:.3f} s, host process {t_proc:.3f} s"),
+        "ms_per_spmv": round(tn * 1e3, 3), "ms_per_spmv_1thread": round(t1 * 1e3, 3),
+    }
+
+
+def extras(torch, dev, steps, warmup):
+    from oracle import pyoracle as O
+    from warpkernel_amd import engine, fe, structure, structure2
+    out = {}
+
+    # FP32-compensated multi-RHS vs plain FP64 (hex elasticity 32^3)
+    m = fe.hex_elastic_3d(32)
+    s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    e64 = engine(s, m.vals, m.colind)
+    vq = m.vals.astype(np.float32)
+    e32 = engine(s, vq.astype(np.float64), m.colind, dtype="f32")
+    rows = []
+    base = None
+    for k in (1, 4, 8, 16, 32):
+        x = fe.uniform_x(m.ncols, k, seed=2100 + k).reshape(m.ncols, k)
+        xq = x.astype(np.float32)
+        dx64, dx32 = dev(x), dev(xq)
+        y64 = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
+        y32 = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
+        _, ms64 = time_launches(torch, lambda: e64.launch(dx64, y64), steps, warmup)
+        _, ms32 = time_launches(torch, lambda: e32.launch(dx32, y32), steps, warmup)
+        _, ms32d = time_launches(torch, lambda: e32.launch(dx32, y32, cfg=dict(accumulate=1)), steps, warmup)
+        e32.launch(dx32, y32)
+        torch.cuda.synchronize()
+        ref_q = O.csr_spmv(m.rowptr, m.colind, vq.astype(np.float64), xq.astype(np.float64))
+        ref_t = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+        yc = y32.cpu().numpy()
+        acc_err = float(np.linalg.norm(yc - ref_q) / np.linalg.norm(ref_q))
+        q_err = float(np.linalg.norm(yc - ref_t) / np.linalg.norm(ref_t))
+        r64 = m.nnz * k / (ms64 * 1e-3)
+        r32 = m.nnz * k / (ms32 * 1e-3)
+        if k == 1:
+            base = r64
+        rows.append({
+            "k": k, "fp64_ms": round(ms64, 4), "fp32c_ms": round(ms32, 4), "fp32_f64acc_ms": round(ms32d, 4),
+            "fp64_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (ms64 * 1e-3) / 1e9, 1),
+            "fp32c_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k, 4, 4, 8) / (ms32 * 1e-3) / 1e9, 1),
+            "fp32c_nnz_rhs_per_s_vs_fp64_k1": round(r32 / base, 3),
+            "fp32c_vs_fp64_same_k": round(ms64 / ms32, 3),
+            "fp32c_err_vs_fp64_oracle_same_inputs": acc_err,
+            "fp32c_err_vs_fp64_unquantised": q_err,
+        })
+    out["multi_rhs_hex_elastic_3d_32"] = rows
+    del e64, e32
+
+    # forward + adjoint (hex diffusion 48^3): y = A x, dx = A^T dy, dvals
+    m = fe.hex_laplace_3d(48)
+    s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind, adjoint=True)
+    fa = {}
+    for k in (1, 8):
+        x = dev(fe.uniform_x(m.ncols, k, seed=2200 + k).reshape(m.ncols, k))
+        dyv = dev(fe.uniform_x(m.nrows, k, seed=3200 + k).reshape(m.nrows, k))
+        y = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
+        dxv = torch.empty((m.ncols, k), dtype=torch.float64, device="cuda")
+        dv = torch.empty(m.nnz, dtype=torch.float64, device="cuda")
+        _, f = time_launches(torch, lambda: eng.launch(x, y), steps, warmup)
+        _, a = time_launches(torch, lambda: eng.run_adjoint(dyv, dxv), steps, warmup)
+        _, d = time_launches(torch, lambda: eng.dvals(dyv, x, dv), steps, warmup)
+        fa[f"k{k}"] = {"fwd_ms": round(f, 4), "dx_ms": round(a, 4), "dvals_ms": round(d, 4),
+                       "fwd_adjoint_ms": round(f + a + d, 4),
+                       "fwd_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (f * 1e-3) / 1e9, 1)}
+    out["fwd_adjoint_hex_laplace_3d_48"] = fa
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--variant", choices=["wpk1", "wpk2"], default="wpk1")
+    ap.add_argument("--threshold", type=int, default=27)
+    ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the WPK SpMV path here has no multi-GPU form "
+              "(how it would shard is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from warpkernel_amd import engine, fe, structure, structure2
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+    print("[bench] building hex_elastic_3d_64", file=sys.stderr, flush=True)
+    m = fe.hex_elastic_3d(64)
+    x = fe.uniform_x(m.ncols, seed=2002)
+    if args.variant == "wpk1":
+        s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    else:
+        s = structure2(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, args.threshold)
+    eng = engine(s, m.vals, m.colind, kr=not args.headline_only)
+    dx = dev(x)
+    dy = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+    print("[bench] timing", file=sys.stderr, flush=True)
+    wall, ev_ms = time_launches(torch, lambda: eng.launch(dx, dy), args.steps, args.warmup)
+    nbytes = spmv_bytes(m.nnz, m.nrows, m.ncols)
+    value = nbytes * args.steps / wall / 1e9
+    achieved = nbytes / (ev_ms * 1e-3) / 1e9
+    # correctness of what was timed
+    from oracle import pyoracle as O
+    ok = bool(np.array_equal(dy.cpu().numpy(), O.csr_spmv(m.rowptr, m.colind, m.vals, x)))
+
+    workload = "hex_elastic_3d_64"
+    traffic = read_traffic(workload)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{workload}: 3D trilinear hex linear elasticity, 64^3 elements, "
+                        f"{m.nrows} rows, {m.nnz} nnz, E~lognormal(0,0.5), nu=0.3, x~U[-1,1]",
+            "kernel": f"{args.variant.upper()} engine::run, group width 64, k=1, FP64 values, int32 indices",
+            "threshold": args.threshold if args.variant == "wpk2" else None,
+            "launch": "default (4 waves/block, unroll 8, XCD remap, nontemporal loads)",
+            "alg_bytes_per_step": nbytes,
+            "gflops": round(2 * m.nnz / (wall / args.steps) / 1e9, 1),
+            "result_bitexact_vs_oracle": ok,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        },
+    }
+    if not args.headline_only:
+        dxr = dev(s.reorder_x(x))
+        dyr = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+        _, msx = time_launches(torch, lambda: eng.launch_x(dxr, dyr), args.steps, args.warmup)
+        ex = {"run_x_GBs": round(nbytes / (msx * 1e-3) / 1e9, 2), "run_x_ms": round(msx, 4)}
+        del eng
+        s2 = structure2(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, args.threshold)
+        e2 = engine(s2, m.vals, m.colind)
+        _, ms2 = time_launches(torch, lambda: e2.launch(dx, dy), args.steps, args.warmup)
+        ex[f"wpk2_t{args.threshold}_GBs"] = round(nbytes / (ms2 * 1e-3) / 1e9, 2)
+        del e2
+        print("[bench] extras", file=sys.stderr, flush=True)
+        ex.update(extras(torch, dev, max(20, args.steps // 5), 10))
+        line["extras"] = ex
+    else:
+        del eng
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(m, x, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

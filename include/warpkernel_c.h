@@ -1,0 +1,205 @@
+/*
+ * warpkernel_c.h -- C ABI of the MI355X (gfx950) WarpKernel SpMV library.
+ *
+ * The reference exposes one C++ class API, namespace warpkernel in
+ * warpkernel.hpp: structure (WPK1 layout), structure2 (WPK2 layout) and
+ * engine<V,I> (device arrays + kernel launches).  Every entry point below names
+ * the reference member it replaces.  include/warpkernel.hpp rebuilds that class
+ * API on top of these functions; Python binds them with ctypes.
+ *
+ * Conventions
+ *   - Every function returns WK_OK (0) or a WK_ERR_* code; wk_last_error()
+ *     returns the message of the last failure on the calling thread.  Nothing
+ *     aborts the process (the reference asserts, warpkernel.hpp:346, 359, 614,
+ *     625, 669, 699).
+ *   - Plain pointers and sizes only.  "host" pointers are CPU memory, "dev"
+ *     pointers are device memory of the current HIP device.  `stream` is a
+ *     hipStream_t passed as void* (NULL = the null stream).
+ *   - Indices are int32 (the reference's IndexType int); nnz < 2^31,
+ *     allocate_nz < 2^32.  warp_offsets are int64.
+ *   - Multi-RHS vectors are row-major: x is [ncols][ldx], y is [nrows][ldy],
+ *     right-hand side j at column j.  k = 1 with ldx = ldy = 1 is a plain vector.
+ *   - No global mutable state: different engines (or the same engine with
+ *     different output buffers) may be used from different streams at once.
+ */
+#ifndef WARPKERNEL_C_H
+#define WARPKERNEL_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WK_ABI_VERSION 1
+
+/* status codes */
+#define WK_OK 0
+#define WK_ERR_ARG 1      /* bad argument (NULL handle, unsupported option) */
+#define WK_ERR_CSR 2      /* malformed CSR input */
+#define WK_ERR_LIMIT 3    /* size exceeds a documented limit */
+#define WK_ERR_HIP 4      /* a HIP runtime call failed */
+#define WK_ERR_STATE 5    /* call not valid in the handle's current state */
+#define WK_ERR_NOMEM 6    /* host allocation failed */
+
+/* value / vector element types */
+#define WK_F64 0          /* FP64 values, x and y */
+#define WK_F32 1          /* FP32 values and x, FP64 y (see accumulate) */
+
+/* accumulation of an FP32 engine */
+#define WK_ACC_TWOSUM 0   /* FP32 TwoProd + TwoSum (compensated), FP64 result */
+#define WK_ACC_F64 1      /* exact FP32 products summed by FP64 fma */
+
+/* layout arrays (wk_layout_array) */
+#define WK_ROW_MAP 0         /* int32 [nrows]      y[row_map[i]] = y'[i]          hpp:276 */
+#define WK_ROW_MAP_INV 1     /* int32 [nrows]                                     hpp:277 */
+#define WK_REORDER_ROWS 2    /* uint32 [nz]        new_values[reorder_rows[i]]=A[i] hpp:278 */
+#define WK_WARP_OFFSETS 3    /* int64 [nwarps+1]                                  hpp:280 */
+#define WK_MAX_NZ_ROWS 4     /* uint32 [nwarps]                                   hpp:281 */
+#define WK_THREADS_PER_ROW 5 /* uint32 [nwarps]    structure2 only                hpp:562 */
+#define WK_ROW_OFFSET_WARP 6 /* uint32 [nwarps+1]  structure2 only                hpp:561 */
+
+/* wk_engine_create flags */
+#define WK_ENGINE_KR 1u       /* also build the Kr (run_x) column indices */
+#define WK_ENGINE_ADJOINT 2u  /* also build the transposed layout (run_adjoint, dvals) */
+
+const char *wk_last_error(void);
+int wk_abi_version(void);
+
+/* ------------------------------------------------------------------ layout */
+
+typedef struct wk_layout wk_layout;
+
+/* Scalars of a scan: structure's public fields (warpkernel.hpp:282-286) and
+ * structure2's (warpkernel.hpp:561-565). */
+typedef struct wk_layout_info {
+  int32_t variant;     /* 1 = structure (WPK1), 2 = structure2 (WPK2) */
+  int32_t group_width; /* W = WARP_SIZE: 64 (one gfx950 wave) or 32 */
+  int32_t threshold;   /* nmax_per_thread (variant 2) */
+  int32_t max_tpr;     /* thmax_per_row, the literal 32 at hpp:643 (variant 2) */
+  int32_t nrows;
+  int32_t ncols;
+  int32_t nznrows;
+  int32_t nwarps;      /* number of groups */
+  int64_t nz;
+  int64_t allocate_nz;
+  uint32_t min_nz;     /* variant 2 */
+  uint32_t max_nz;     /* variant 2 */
+} wk_layout_info;
+
+/* structure::scan (warpkernel.hpp:309-365).  Only rowptr shapes the layout;
+ * colind (may be NULL) is range-checked against ncols when given. */
+int wk_scan1(int32_t group_width, int32_t nrows, int32_t ncols, int64_t nnz,
+             const int32_t *rowptr_host, const int32_t *colind_host, wk_layout **out);
+
+/* structure2::scan(..., threshold) (warpkernel.hpp:578-736).  max_tpr is a
+ * power of two in [2, group_width]; 32 reproduces the reference. */
+int wk_scan2(int32_t group_width, int32_t threshold, int32_t max_tpr, int32_t nrows,
+             int32_t ncols, int64_t nnz, const int32_t *rowptr_host,
+             const int32_t *colind_host, wk_layout **out);
+
+int wk_layout_get_info(const wk_layout *layout, wk_layout_info *info);
+
+/* Borrowed pointer to one of the arrays above (valid until destroy). */
+int wk_layout_array(const wk_layout *layout, int32_t which, const void **data, int64_t *count);
+
+/* structure::reorder (warpkernel.hpp:379-398), host side: zero-filled padded
+ * arrays of allocate_nz entries, pvals[reorder_rows[i]] = vals[i]. */
+int wk_layout_process_host(const wk_layout *layout, const double *vals_host,
+                           const int32_t *colind_host, double *pvals_host, int32_t *pcols_host);
+
+/* structure::reorder_columns_coalesced (warpkernel.hpp:446-457). */
+int wk_layout_reorder_columns_coalesced(const wk_layout *layout, const int32_t *colind_host,
+                                        int32_t *pcols_kr_host);
+
+/* structure::reorder_x (warpkernel.hpp:517-530): xout[row_map_inv[r]] = x[r]. */
+int wk_layout_reorder_x(const wk_layout *layout, const double *x_host, double *xout_host);
+
+void wk_layout_destroy(wk_layout *layout);
+
+/* ------------------------------------------------------------------ engine */
+
+typedef struct wk_engine wk_engine;
+
+/* engine(structure&, A, colinds) / engine(structure2&, A, colinds)
+ * (warpkernel.hpp:794-809) -> structure::process (hpp:532-548) and
+ * structure2::process (hpp:749-769): the padded arrays and the layout metadata
+ * are built and copied to the current device.  vals_host is FP64 CSR (cast to
+ * FP32 when values_dtype is WK_F32).  The engine copies what it needs; the
+ * layout may be destroyed afterwards. */
+int wk_engine_create(const wk_layout *layout, const int32_t *rowptr_host,
+                     const int32_t *colind_host, const double *vals_host, int32_t values_dtype,
+                     uint32_t flags, void *stream, wk_engine **out);
+
+void wk_engine_destroy(wk_engine *engine);
+
+int wk_engine_get_info(const wk_engine *engine, wk_layout_info *info);
+
+/* GPU-side process: replace the values, keeping the pattern.  vals_dev is CSR
+ * order on the device (FP64, or FP32 for a WK_F32 engine), scattered into the
+ * padded layout(s) on `stream` (the thrust::scatter of warpkernel.hpp:459-502,
+ * without atomics). */
+int wk_engine_update_values(wk_engine *engine, const void *vals_dev, void *stream);
+
+/* The reference's drivers overwrite the public engine::device_colinds with the
+ * Kr columns and restore them (wpk1.cu:160-197).  pcols has allocate_nz
+ * entries in padded order; where = 0 host, 1 device. */
+int wk_engine_set_colinds(wk_engine *engine, const int32_t *pcols, int32_t where, void *stream);
+int wk_engine_get_colinds(const wk_engine *engine, int32_t *pcols_host);
+
+/* Launch options; wk_launch_default fills the built-in choice. */
+typedef struct wk_launch {
+  int32_t waves_per_block; /* 1, 2, 4 or 8 (the drivers' warps/block sweep) */
+  int32_t unroll;          /* slots in flight per lane: 4, 8 or 16 (k_tile 1) */
+  int32_t k_tile;          /* right-hand sides per pass: 1, 2, 4, 8, 16 or 32 */
+  int32_t xcd_remap;       /* 1: contiguous group ranges per XCD */
+  int32_t nontemporal;     /* 1: streaming loads for values and indices */
+  int32_t accumulate;      /* WK_ACC_* for FP32 engines */
+} wk_launch;
+
+void wk_launch_default(wk_launch *cfg);
+
+/* engine::run (warpkernel.hpp:811-860): y[row_map[i]] = sum over group slots.
+ * Asynchronous on `stream`.  Rows without nonzeros are written as 0 (the
+ * reference leaves them untouched and its drivers pre-zero y, wpk1.cu:146).
+ * x: [ncols][ldx] (FP64, or FP32 for a WK_F32 engine); y: FP64 [nrows][ldy].
+ * cfg may be NULL. */
+int wk_run(wk_engine *engine, const wk_launch *cfg, const void *x_dev, double *y_dev,
+           int32_t k, int64_t ldx, int64_t ldy, void *stream);
+
+/* engine::run_x (warpkernel.hpp:862-896), the Kr form: y'[i] = sum, with x and
+ * the column indices in the permuted order (reorder_x, reorder_columns_coalesced).
+ * Uses the Kr columns built by WK_ENGINE_KR (or set by the caller through
+ * wk_engine_set_colinds when the engine has none). */
+int wk_run_x(wk_engine *engine, const wk_launch *cfg, const void *x_dev, double *y_dev,
+             int32_t k, int64_t ldx, int64_t ldy, void *stream);
+
+/* r = b - A x in FP64 (FP64 engines).  No reference counterpart. */
+int wk_residual(wk_engine *engine, const wk_launch *cfg, const double *b_dev, int64_t ldb,
+                const double *x_dev, double *r_dev, int32_t k, int64_t ldx, int64_t ldr,
+                void *stream);
+
+/* dx = A^T dy through the transposed WPK layout (WK_ENGINE_ADJOINT); no
+ * atomics.  dy: [nrows][lddy], dx: [ncols][lddx].  No reference counterpart. */
+int wk_run_adjoint(wk_engine *engine, const wk_launch *cfg, const void *dy_dev, double *dx_dev,
+                   int32_t k, int64_t lddy, int64_t lddx, void *stream);
+
+/* dvals[j] = sum_r dy[row_j][r] * x[col_j][r] in CSR order (FP64): the
+ * gradient of y = A(vals) x with respect to the values.  No reference
+ * counterpart. */
+int wk_dvals(wk_engine *engine, const double *dy_dev, int64_t lddy, const double *x_dev,
+             int64_t ldx, int32_t k, double *dvals_dev, void *stream);
+
+/* Blocking timing helper with the reference's contract (engine::run returns
+ * the seconds of one launch, warpkernel.hpp:829-843): launches once, waits,
+ * returns the elapsed seconds measured with HIP events in *seconds.
+ * which: 0 run, 1 run_x. */
+int wk_time_run(wk_engine *engine, const wk_launch *cfg, int32_t which, const void *x_dev,
+                double *y_dev, int32_t k, int64_t ldx, int64_t ldy, void *stream,
+                float *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

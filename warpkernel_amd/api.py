@@ -1,0 +1,318 @@
+"""Python mirror of the reference's class API (namespace warpkernel,
+warpkernel.hpp:215-959) over the C ABI.
+
+    s = structure();  s.scan(nz, nrows, A, rows, colinds)          # hpp:309-365
+    s2 = structure2(); s2.scan(nz, nrows, A, rows, colinds, th)    # hpp:578-736
+    eng = engine(s, A, colinds)                                    # hpp:794-809
+    t = eng.run(nblocks, blocksize, dx, dy)                        # hpp:811-860
+    t = eng.run_x(nblocks, blocksize, dxr, dy)                     # hpp:862-896
+    eng.verify(y_ref, y)                                           # hpp:932-943
+
+Differences, all deliberate:
+  - group_width (the reference's WARP_SIZE) defaults to 64, one gfx950 wave;
+    32 reproduces the reference's layout exactly.
+  - dx / dy are torch device tensors (or raw device pointers as ints); run and
+    run_x return seconds and block, as the reference's do; launch / launch_x
+    are the asynchronous forms with k right-hand sides.
+  - Rows without nonzeros are written as 0.
+  - verify_x compares orig[row_map[i]] with comp[i] in both of its tests (the
+    reference indexes orig[i] in the zero test, hpp:949-950).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+from ._capi import check, lib
+
+
+def _host_ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dev_ptr(t) -> int:
+    if t is None:
+        return 0
+    if isinstance(t, int):
+        return t
+    return int(t.data_ptr())
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        import torch
+        return int(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+class structure:
+    """WPK1 layout (warpkernel::structure, warpkernel.hpp:215-551)."""
+
+    variant = 1
+
+    def __init__(self, group_width: int = 64):
+        self.group_width = int(group_width)
+        self._h = None
+        self._rowptr = None
+        self.nrows = self.ncols = self.nznrows = self.nz = self.allocate_nz = self.nwarps = 0
+
+    # -- scan -----------------------------------------------------------
+    def _scan_call(self, nz, nrows, rows, colinds, ncols, extra):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = None if colinds is None else np.ascontiguousarray(colinds, dtype=np.int32)
+        if rows.size != int(nrows) + 1:
+            raise ValueError("rows must hold nrows + 1 offsets")
+        ncols = int(nrows) if ncols is None else int(ncols)
+        h = C.c_void_p()
+        cptr = None if cols is None else _host_ptr(cols)
+        if self.variant == 1:
+            check(lib().wk_scan1(self.group_width, int(nrows), ncols, int(nz), _host_ptr(rows), cptr, C.byref(h)))
+        else:
+            check(lib().wk_scan2(self.group_width, extra[0], extra[1], int(nrows), ncols, int(nz),
+                                 _host_ptr(rows), cptr, C.byref(h)))
+        self.close()
+        self._h = h
+        self._rowptr = rows
+        self._load_fields()
+
+    def scan(self, nz, nrows, A, rows, colinds, ncols=None):
+        """structure::scan(nz, nrows, A, rows, colinds) -- only rows shapes the
+        layout; colinds is range-checked.  ncols defaults to nrows."""
+        self._scan_call(nz, nrows, rows, colinds, ncols, None)
+        return self
+
+    def _array(self, which, dtype):
+        data = C.c_void_p()
+        n = C.c_int64()
+        check(lib().wk_layout_array(self._h, which, C.byref(data), C.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, dtype=dtype)
+        buf = (C.c_char * (n.value * np.dtype(dtype).itemsize)).from_address(data.value)
+        return np.frombuffer(buf, dtype=dtype).copy()
+
+    def _load_fields(self):
+        info = capi.LayoutInfo()
+        check(lib().wk_layout_get_info(self._h, C.byref(info)))
+        self.info = info
+        self.nrows, self.ncols, self.nznrows = info.nrows, info.ncols, info.nznrows
+        self.nz, self.allocate_nz, self.nwarps = info.nz, info.allocate_nz, info.nwarps
+        self.row_map = self._array(capi.WK_ROW_MAP, np.int32)
+        self.row_map_inv = self._array(capi.WK_ROW_MAP_INV, np.int32)
+        self.reorder_rows = self._array(capi.WK_REORDER_ROWS, np.uint32)
+        self.warp_offsets = self._array(capi.WK_WARP_OFFSETS, np.int64)
+        self.max_nz_rows = self._array(capi.WK_MAX_NZ_ROWS, np.uint32)
+
+    # -- host helpers ---------------------------------------------------
+    def process_host(self, A, colinds):
+        """structure::reorder (hpp:379-398) on the host: padded (values, colinds)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        colinds = np.ascontiguousarray(colinds, dtype=np.int32)
+        pv = np.empty(self.allocate_nz, np.float64)
+        pc = np.empty(self.allocate_nz, np.int32)
+        check(lib().wk_layout_process_host(self._h, _host_ptr(A), _host_ptr(colinds), _host_ptr(pv), _host_ptr(pc)))
+        return pv, pc
+
+    def reorder_columns_coalesced(self, colinds):
+        """structure::reorder_columns_coalesced (hpp:446-457): padded Kr columns."""
+        colinds = np.ascontiguousarray(colinds, dtype=np.int32)
+        out = np.empty(self.allocate_nz, np.int32)
+        check(lib().wk_layout_reorder_columns_coalesced(self._h, _host_ptr(colinds), _host_ptr(out)))
+        return out
+
+    def reorder_x(self, x):
+        """structure::reorder_x (hpp:517-530): x'[row_map_inv[r]] = x[r]."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty_like(x)
+        check(lib().wk_layout_reorder_x(self._h, _host_ptr(x), _host_ptr(out)))
+        return out
+
+    def close(self):
+        if self._h is not None:
+            lib().wk_layout_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class structure2(structure):
+    """WPK2 layout (warpkernel::structure2, warpkernel.hpp:553-771)."""
+
+    variant = 2
+
+    def __init__(self, group_width: int = 64, max_tpr: int = 32):
+        super().__init__(group_width)
+        self.max_tpr = int(max_tpr)
+        self.nmax_per_thread = 0
+
+    def scan(self, nz, nrows, A, rows, colinds, threshold, ncols=None):
+        """structure2::scan(nz, nrows, A, rows, colinds, threshold)."""
+        self.nmax_per_thread = int(threshold)
+        self._scan_call(nz, nrows, rows, colinds, ncols, (int(threshold), self.max_tpr))
+        return self
+
+    def _load_fields(self):
+        super()._load_fields()
+        self.min_nz, self.max_nz = self.info.min_nz, self.info.max_nz
+        self.threads_per_row = self._array(capi.WK_THREADS_PER_ROW, np.uint32)
+        self.row_offset_warp = self._array(capi.WK_ROW_OFFSET_WARP, np.uint32)
+
+
+class engine:
+    """Device arrays + kernel launches (warpkernel::engine, warpkernel.hpp:773-959).
+
+    engine(k, A, colinds) runs process (hpp:794-809).  dtype 'f64' or 'f32'
+    (FP32 values and x, FP64 y); kr=True also builds the Kr columns for run_x;
+    adjoint=True builds the transposed layout for run_adjoint and dvals.
+    """
+
+    def __init__(self, k: structure, A, colinds, dtype: str = "f64", kr: bool = False,
+                 adjoint: bool = False, stream=None):
+        if k._h is None:
+            raise ValueError("structure has not been scanned")
+        self.kernel = k
+        self.dtype = {"f64": capi.WK_F64, "f32": capi.WK_F32}[dtype]
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        colinds = np.ascontiguousarray(colinds, dtype=np.int32)
+        flags = (capi.WK_ENGINE_KR if kr else 0) | (capi.WK_ENGINE_ADJOINT if adjoint else 0)
+        h = C.c_void_p()
+        check(lib().wk_engine_create(k._h, _host_ptr(k._rowptr), _host_ptr(colinds), _host_ptr(A),
+                                     self.dtype, flags, _stream(stream), C.byref(h)))
+        self._h = h
+        self.nrows, self.ncols, self.nz = k.nrows, k.ncols, k.nz
+        self.has_kr, self.has_adjoint = kr, adjoint
+
+    # -- reference-shaped, blocking, returns seconds --------------------
+    def _cfg(self, blocksize, cfg):
+        c = capi.default_launch(**(cfg or {}))
+        if blocksize:
+            c.waves_per_block = max(1, min(8, int(blocksize) // 64))
+        return c
+
+    def run(self, nblocks, blocksize, dx, dy, cfg=None, stream=None) -> float:
+        """engine::run<cache>(nblocks, blocksize, dx, dy): seconds of one launch.
+        The grid is derived from the layout; nblocks is accepted for the
+        reference's signature.  blocksize / 64 = waves per block (1..8)."""
+        del nblocks
+        sec = C.c_float()
+        c = self._cfg(blocksize, cfg)
+        check(lib().wk_time_run(self._h, C.byref(c), 0, _dev_ptr(dx), _dev_ptr(dy), 1, 1, 1,
+                                _stream(stream), C.byref(sec)))
+        return sec.value
+
+    def run_x(self, nblocks, blocksize, dx, dy, cfg=None, stream=None) -> float:
+        """engine::run_x<cache>: the Kr form, y'[i] for x' = reorder_x(x)."""
+        del nblocks
+        sec = C.c_float()
+        c = self._cfg(blocksize, cfg)
+        check(lib().wk_time_run(self._h, C.byref(c), 1, _dev_ptr(dx), _dev_ptr(dy), 1, 1, 1,
+                                _stream(stream), C.byref(sec)))
+        return sec.value
+
+    # -- asynchronous forms ---------------------------------------------
+    @staticmethod
+    def _k_ld(t):
+        if t is None or isinstance(t, int):
+            return 1, 1
+        if t.dim() == 1:
+            return 1, 1
+        return int(t.shape[1]), int(t.stride(0))
+
+    def launch(self, x, y, cfg=None, stream=None):
+        """y = A x (k right-hand sides if x, y are [n, k] row-major)."""
+        k, ldx = self._k_ld(x)
+        _, ldy = self._k_ld(y)
+        c = capi.default_launch(**(cfg or {}))
+        check(lib().wk_run(self._h, C.byref(c), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, _stream(stream)))
+
+    def launch_x(self, x, y, cfg=None, stream=None):
+        k, ldx = self._k_ld(x)
+        _, ldy = self._k_ld(y)
+        c = capi.default_launch(**(cfg or {}))
+        check(lib().wk_run_x(self._h, C.byref(c), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, _stream(stream)))
+
+    def residual(self, b, x, r, cfg=None, stream=None):
+        """r = b - A x in FP64."""
+        k, ldx = self._k_ld(x)
+        _, ldb = self._k_ld(b)
+        _, ldr = self._k_ld(r)
+        c = capi.default_launch(**(cfg or {}))
+        check(lib().wk_residual(self._h, C.byref(c), _dev_ptr(b), ldb, _dev_ptr(x), _dev_ptr(r), k,
+                                ldx, ldr, _stream(stream)))
+
+    def run_adjoint(self, dy, dx, cfg=None, stream=None):
+        """dx = A^T dy through the transposed layout."""
+        k, lddy = self._k_ld(dy)
+        _, lddx = self._k_ld(dx)
+        c = capi.default_launch(**(cfg or {}))
+        check(lib().wk_run_adjoint(self._h, C.byref(c), _dev_ptr(dy), _dev_ptr(dx), k, lddy, lddx,
+                                   _stream(stream)))
+
+    def dvals(self, dy, x, out, stream=None):
+        """out[j] = sum_r dy[row_j, r] * x[col_j, r] (CSR order)."""
+        k, lddy = self._k_ld(dy)
+        _, ldx = self._k_ld(x)
+        check(lib().wk_dvals(self._h, _dev_ptr(dy), lddy, _dev_ptr(x), ldx, k, _dev_ptr(out), _stream(stream)))
+
+    def update_values(self, vals_dev, stream=None):
+        """GPU-side process of new CSR values (same pattern)."""
+        check(lib().wk_engine_update_values(self._h, _dev_ptr(vals_dev), _stream(stream)))
+
+    @property
+    def device_colinds(self) -> np.ndarray:
+        out = np.empty(self.kernel.allocate_nz, np.int32)
+        check(lib().wk_engine_get_colinds(self._h, _host_ptr(out)))
+        return out
+
+    @device_colinds.setter
+    def device_colinds(self, pcols):
+        """eng.device_colinds = reorder_cols (wpk1.cu:175): host array or device tensor."""
+        if isinstance(pcols, np.ndarray):
+            pcols = np.ascontiguousarray(pcols, dtype=np.int32)
+            if pcols.size != self.kernel.allocate_nz:
+                raise ValueError("padded column array has the wrong length")
+            check(lib().wk_engine_set_colinds(self._h, _host_ptr(pcols), 0, _stream(None)))
+        else:
+            check(lib().wk_engine_set_colinds(self._h, _dev_ptr(pcols), 1, _stream(None)))
+
+    # -- verification (host) ------------------------------------------------
+    def verify(self, orig, comp) -> bool:
+        """engine::verify (hpp:932-943): relative 1e-5 where |orig| > 1e-14,
+        absolute 1e-14 elsewhere; stops at the first failure."""
+        return _verify(np.asarray(orig), np.asarray(comp), None)
+
+    def verify_x(self, orig, comp) -> bool:
+        """engine::verify_x (hpp:946-957): compares orig[row_map[i]] with comp[i]."""
+        return _verify(np.asarray(orig), np.asarray(comp), self.kernel.row_map)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().wk_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _verify(orig, comp, row_map) -> bool:
+    o = orig if row_map is None else orig[row_map]
+    c = comp
+    small = np.abs(o) <= 1e-14
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = np.abs((o - c) / o)
+    bad = (small & (np.abs(o - c) > 1e-14)) | (~small & (rel > 1e-5))
+    if bad.any():
+        i = int(np.argmax(bad))
+        print(f"verify: {o[i]}\t{c[i]}\t{i}")
+        return False
+    return True

@@ -1,0 +1,426 @@
+// capi.cpp -- the extern "C" boundary (include/warpkernel_c.h): error codes,
+// the host layout object (structure / structure2) and the device engine
+// (engine<V,I>, warpkernel.hpp:773-959).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+
+#include "wk_internal.hpp"
+
+namespace wk {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) fail(WK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return WK_OK;
+  } catch (const Error &e) {
+    set_last_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return WK_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return WK_ERR_ARG;
+  }
+}
+
+template <class T>
+T *dev_alloc(wk_engine *E, int64_t count) {
+  if (count <= 0) return nullptr;
+  void *p = nullptr;
+  hip_check(hipMalloc(&p, sizeof(T) * (size_t)count), "hipMalloc");
+  E->owned.push_back(p);
+  return static_cast<T *>(p);
+}
+
+template <class T>
+T *dev_upload(wk_engine *E, const T *src, int64_t count, hipStream_t s) {
+  T *d = dev_alloc<T>(E, count);
+  if (d) hip_check(hipMemcpyAsync(d, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, s), "upload");
+  return d;
+}
+
+void free_engine(wk_engine *E) {
+  if (!E) return;
+  // teardown is best effort: errors here have nowhere to go
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(E->device);
+  for (void *p : E->owned) (void)hipFree(p);
+  (void)hipSetDevice(cur);
+  delete E;
+}
+
+// Copies one host layout (and its padded values/columns) to the device.
+// vals/colind are the layout's own CSR; slot_of_nz (optional) replaces
+// reorder_rows as the device scatter map (the transpose is scattered from A's
+// CSR order).
+void upload_layout(wk_engine *E, DevLayout &D, const wk_layout &L, const double *vals,
+                   const int32_t *colind, int32_t dtype, bool kr, const uint32_t *slot_of_nz,
+                   hipStream_t s) {
+  D.variant = L.variant;
+  D.W = L.W;
+  D.nrows = L.nrows;
+  D.nznrows = L.nznrows;
+  D.ngroups = L.nwarps;
+  D.allocate_nz = L.allocate_nz;
+  std::vector<int32_t> pcols((size_t)L.allocate_nz);
+  if (dtype == WK_F64) {
+    std::vector<double> pv((size_t)L.allocate_nz);
+    process_host<double>(L, vals, colind, pv.data(), pcols.data());
+    D.vals = dev_upload(E, pv.data(), L.allocate_nz, s);
+    hip_check(hipStreamSynchronize(s), "upload");
+  } else {
+    std::vector<float> pv((size_t)L.allocate_nz);
+    process_host<float>(L, vals, colind, pv.data(), pcols.data());
+    D.vals = dev_upload(E, pv.data(), L.allocate_nz, s);
+    hip_check(hipStreamSynchronize(s), "upload");
+  }
+  D.cols = dev_upload(E, pcols.data(), L.allocate_nz, s);
+  hip_check(hipStreamSynchronize(s), "upload");
+  if (kr) {
+    reorder_columns_coalesced(L, colind, pcols.data());
+    D.cols_kr = dev_upload(E, pcols.data(), L.allocate_nz, s);
+    hip_check(hipStreamSynchronize(s), "upload");
+  }
+  D.row_map = dev_upload(E, L.row_map.data(), L.nrows, s);
+  D.warp_offsets = dev_upload(E, L.warp_offsets.data(), (int64_t)L.warp_offsets.size(), s);
+  D.max_nz_rows = dev_upload(E, L.max_nz_rows.data(), (int64_t)L.max_nz_rows.size(), s);
+  if (L.variant == 2) {
+    D.tpr = dev_upload(E, L.threads_per_row.data(), (int64_t)L.threads_per_row.size(), s);
+    D.row_off = dev_upload(E, L.row_offset_warp.data(), (int64_t)L.row_offset_warp.size(), s);
+  }
+  D.slot_of_nz = dev_upload(E, slot_of_nz ? slot_of_nz : L.reorder_rows.data(), L.nz, s);
+  hip_check(hipStreamSynchronize(s), "upload");
+}
+
+wk_launch resolve(const wk_launch *cfg) {
+  wk_launch c;
+  wk_launch_default(&c);
+  if (cfg) c = *cfg;
+  return c;
+}
+
+void check_engine(const wk_engine *E) {
+  if (!E) fail(WK_ERR_ARG, "engine is NULL");
+}
+
+}  // namespace
+}  // namespace wk
+
+using namespace wk;
+
+extern "C" {
+
+const char *wk_last_error(void) { return g_last_error.c_str(); }
+
+int wk_abi_version(void) { return WK_ABI_VERSION; }
+
+int wk_scan1(int32_t group_width, int32_t nrows, int32_t ncols, int64_t nnz,
+             const int32_t *rowptr_host, const int32_t *colind_host, wk_layout **out) {
+  return guarded([&] {
+    if (!out) fail(WK_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    validate_csr(nrows, ncols, nnz, rowptr_host, colind_host);
+    auto *L = new wk_layout();
+    try {
+      scan1(*L, group_width, nrows, ncols, nnz, rowptr_host);
+    } catch (...) {
+      delete L;
+      throw;
+    }
+    *out = L;
+  });
+}
+
+int wk_scan2(int32_t group_width, int32_t threshold, int32_t max_tpr, int32_t nrows,
+             int32_t ncols, int64_t nnz, const int32_t *rowptr_host, const int32_t *colind_host,
+             wk_layout **out) {
+  return guarded([&] {
+    if (!out) fail(WK_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    validate_csr(nrows, ncols, nnz, rowptr_host, colind_host);
+    auto *L = new wk_layout();
+    try {
+      scan2(*L, group_width, threshold, max_tpr, nrows, ncols, nnz, rowptr_host);
+    } catch (...) {
+      delete L;
+      throw;
+    }
+    *out = L;
+  });
+}
+
+int wk_layout_get_info(const wk_layout *L, wk_layout_info *info) {
+  return guarded([&] {
+    if (!L || !info) fail(WK_ERR_ARG, "NULL argument");
+    info->variant = L->variant;
+    info->group_width = L->W;
+    info->threshold = L->threshold;
+    info->max_tpr = L->max_tpr;
+    info->nrows = L->nrows;
+    info->ncols = L->ncols;
+    info->nznrows = L->nznrows;
+    info->nwarps = L->nwarps;
+    info->nz = L->nz;
+    info->allocate_nz = L->allocate_nz;
+    info->min_nz = L->min_nz;
+    info->max_nz = L->max_nz;
+  });
+}
+
+int wk_layout_array(const wk_layout *L, int32_t which, const void **data, int64_t *count) {
+  return guarded([&] {
+    if (!L || !data || !count) fail(WK_ERR_ARG, "NULL argument");
+    switch (which) {
+      case WK_ROW_MAP: *data = L->row_map.data(); *count = (int64_t)L->row_map.size(); break;
+      case WK_ROW_MAP_INV: *data = L->row_map_inv.data(); *count = (int64_t)L->row_map_inv.size(); break;
+      case WK_REORDER_ROWS: *data = L->reorder_rows.data(); *count = (int64_t)L->reorder_rows.size(); break;
+      case WK_WARP_OFFSETS: *data = L->warp_offsets.data(); *count = (int64_t)L->warp_offsets.size(); break;
+      case WK_MAX_NZ_ROWS: *data = L->max_nz_rows.data(); *count = (int64_t)L->max_nz_rows.size(); break;
+      case WK_THREADS_PER_ROW:
+        if (L->variant != 2) fail(WK_ERR_STATE, "threads_per_row exists for structure2 only");
+        *data = L->threads_per_row.data(); *count = (int64_t)L->threads_per_row.size(); break;
+      case WK_ROW_OFFSET_WARP:
+        if (L->variant != 2) fail(WK_ERR_STATE, "row_offset_warp exists for structure2 only");
+        *data = L->row_offset_warp.data(); *count = (int64_t)L->row_offset_warp.size(); break;
+      default: fail(WK_ERR_ARG, "unknown layout array");
+    }
+  });
+}
+
+int wk_layout_process_host(const wk_layout *L, const double *vals_host, const int32_t *colind_host,
+                           double *pvals_host, int32_t *pcols_host) {
+  return guarded([&] {
+    if (!L || (!pvals_host && !pcols_host)) fail(WK_ERR_ARG, "NULL argument");
+    if ((pvals_host && !vals_host) || (pcols_host && !colind_host)) fail(WK_ERR_ARG, "NULL input");
+    process_host<double>(*L, vals_host, colind_host, pvals_host, pcols_host);
+  });
+}
+
+int wk_layout_reorder_columns_coalesced(const wk_layout *L, const int32_t *colind_host,
+                                        int32_t *pcols_kr_host) {
+  return guarded([&] {
+    if (!L || !colind_host || !pcols_kr_host) fail(WK_ERR_ARG, "NULL argument");
+    if (L->nrows != L->ncols) fail(WK_ERR_ARG, "the Kr form needs a square matrix");
+    reorder_columns_coalesced(*L, colind_host, pcols_kr_host);
+  });
+}
+
+int wk_layout_reorder_x(const wk_layout *L, const double *x_host, double *xout_host) {
+  return guarded([&] {
+    if (!L || !x_host || !xout_host) fail(WK_ERR_ARG, "NULL argument");
+    for (int32_t r = 0; r < L->nrows; r++) xout_host[L->row_map_inv[r]] = x_host[r];
+  });
+}
+
+void wk_layout_destroy(wk_layout *L) { delete L; }
+
+int wk_engine_create(const wk_layout *L, const int32_t *rowptr_host, const int32_t *colind_host,
+                     const double *vals_host, int32_t values_dtype, uint32_t flags, void *stream,
+                     wk_engine **out) {
+  return guarded([&] {
+    if (!out) fail(WK_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!L || !rowptr_host || !colind_host || !vals_host) fail(WK_ERR_ARG, "NULL argument");
+    if (values_dtype != WK_F64 && values_dtype != WK_F32) fail(WK_ERR_ARG, "unknown values dtype");
+    if (values_dtype == WK_F32 && L->W != 64) fail(WK_ERR_ARG, "FP32 engines need group_width 64");
+    if (rowptr_host[L->nrows] != L->nz) fail(WK_ERR_CSR, "rowptr does not match the layout");
+    const bool kr = flags & WK_ENGINE_KR, adj = flags & WK_ENGINE_ADJOINT;
+    if (kr && L->nrows != L->ncols) fail(WK_ERR_ARG, "the Kr form needs a square matrix");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto *E = new wk_engine();
+    try {
+      hip_check(hipGetDevice(&E->device), "hipGetDevice");
+      E->dtype = values_dtype;
+      E->nrows = L->nrows;
+      E->ncols = L->ncols;
+      E->nnz = L->nz;
+      wk_layout_get_info(L, &E->info);
+      upload_layout(E, E->fwd, *L, vals_host, colind_host, values_dtype, kr, nullptr, s);
+      if (adj) {
+        // A^T in CSR with the same scan options; its values are A's values
+        // permuted, and slot_of_nz maps A's CSR index to the transposed slot.
+        std::vector<int32_t> colptr, rowind;
+        std::vector<int64_t> perm;
+        transpose_csr(L->nrows, L->ncols, rowptr_host, colind_host, colptr, rowind, perm);
+        std::vector<double> tv(perm.size());
+        for (size_t t = 0; t < perm.size(); t++) tv[t] = vals_host[perm[t]];
+        wk_layout LT;
+        if (L->variant == 2)
+          scan2(LT, L->W, L->threshold, L->max_tpr, L->ncols, L->nrows, L->nz, colptr.data());
+        else
+          scan1(LT, L->W, L->ncols, L->nrows, L->nz, colptr.data());
+        std::vector<uint32_t> slot_t((size_t)L->nz);
+        for (size_t t = 0; t < perm.size(); t++) slot_t[perm[t]] = LT.reorder_rows[t];
+        upload_layout(E, E->adj, LT, tv.data(), rowind.data(), values_dtype, false, slot_t.data(), s);
+        E->has_adj = true;
+        E->d_rowptr = dev_upload(E, rowptr_host, (int64_t)L->nrows + 1, s);
+        E->d_colind = dev_upload(E, colind_host, L->nz, s);
+        hip_check(hipStreamSynchronize(s), "upload");
+      }
+    } catch (...) {
+      free_engine(E);
+      throw;
+    }
+    *out = E;
+  });
+}
+
+void wk_engine_destroy(wk_engine *E) { free_engine(E); }
+
+int wk_engine_get_info(const wk_engine *E, wk_layout_info *info) {
+  return guarded([&] {
+    check_engine(E);
+    if (!info) fail(WK_ERR_ARG, "info is NULL");
+    *info = E->info;
+  });
+}
+
+int wk_engine_update_values(wk_engine *E, const void *vals_dev, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (!vals_dev && E->nnz) fail(WK_ERR_ARG, "vals_dev is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    launch_scatter_values(E->fwd, E->dtype, E->nnz, vals_dev, s);
+    if (E->has_adj) launch_scatter_values(E->adj, E->dtype, E->nnz, vals_dev, s);
+  });
+}
+
+int wk_engine_set_colinds(wk_engine *E, const int32_t *pcols, int32_t where, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (!pcols && E->fwd.allocate_nz) fail(WK_ERR_ARG, "pcols is NULL");
+    const size_t bytes = sizeof(int32_t) * (size_t)E->fwd.allocate_nz;
+    if (!bytes) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hip_check(hipMemcpyAsync(E->fwd.cols, pcols, bytes,
+                             where ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s),
+              "set_colinds");
+    if (!where) hip_check(hipStreamSynchronize(s), "set_colinds");
+  });
+}
+
+int wk_engine_get_colinds(const wk_engine *E, int32_t *pcols_host) {
+  return guarded([&] {
+    check_engine(E);
+    const size_t bytes = sizeof(int32_t) * (size_t)E->fwd.allocate_nz;
+    if (!bytes) return;
+    if (!pcols_host) fail(WK_ERR_ARG, "pcols_host is NULL");
+    hip_check(hipMemcpy(pcols_host, E->fwd.cols, bytes, hipMemcpyDeviceToHost), "get_colinds");
+  });
+}
+
+void wk_launch_default(wk_launch *cfg) {
+  if (!cfg) return;
+  cfg->waves_per_block = 4;
+  cfg->unroll = 8;
+  cfg->k_tile = 8;
+  cfg->xcd_remap = 1;
+  cfg->nontemporal = 1;
+  cfg->accumulate = WK_ACC_TWOSUM;
+}
+
+int wk_run(wk_engine *E, const wk_launch *cfg, const void *x_dev, double *y_dev, int32_t k,
+           int64_t ldx, int64_t ldy, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if ((!x_dev && E->ncols) || (!y_dev && E->nrows)) fail(WK_ERR_ARG, "NULL vector");
+    if (ldx < k || ldy < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
+    launch_spmv(E->fwd, E->dtype, resolve(cfg), OUT_RUN, x_dev, ldx, y_dev, ldy, nullptr, 0, k,
+                static_cast<hipStream_t>(stream));
+  });
+}
+
+int wk_run_x(wk_engine *E, const wk_launch *cfg, const void *x_dev, double *y_dev, int32_t k,
+             int64_t ldx, int64_t ldy, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (E->nrows != E->ncols) fail(WK_ERR_ARG, "the Kr form needs a square matrix");
+    if ((!x_dev && E->ncols) || (!y_dev && E->nrows)) fail(WK_ERR_ARG, "NULL vector");
+    if (ldx < k || ldy < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
+    launch_spmv(E->fwd, E->dtype, resolve(cfg), OUT_RUNX, x_dev, ldx, y_dev, ldy, nullptr, 0, k,
+                static_cast<hipStream_t>(stream));
+  });
+}
+
+int wk_residual(wk_engine *E, const wk_launch *cfg, const double *b_dev, int64_t ldb,
+                const double *x_dev, double *r_dev, int32_t k, int64_t ldx, int64_t ldr,
+                void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (E->dtype != WK_F64) fail(WK_ERR_ARG, "the residual is computed by FP64 engines");
+    if ((!x_dev && E->ncols) || ((!r_dev || !b_dev) && E->nrows)) fail(WK_ERR_ARG, "NULL vector");
+    if (ldx < k || ldr < k || ldb < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
+    launch_spmv(E->fwd, E->dtype, resolve(cfg), OUT_RESID, x_dev, ldx, r_dev, ldr, b_dev, ldb, k,
+                static_cast<hipStream_t>(stream));
+  });
+}
+
+int wk_run_adjoint(wk_engine *E, const wk_launch *cfg, const void *dy_dev, double *dx_dev,
+                   int32_t k, int64_t lddy, int64_t lddx, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (!E->has_adj) fail(WK_ERR_STATE, "engine was created without WK_ENGINE_ADJOINT");
+    if ((!dy_dev && E->nrows) || (!dx_dev && E->ncols)) fail(WK_ERR_ARG, "NULL vector");
+    if (lddy < k || lddx < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
+    launch_spmv(E->adj, E->dtype, resolve(cfg), OUT_RUN, dy_dev, lddy, dx_dev, lddx, nullptr, 0,
+                k, static_cast<hipStream_t>(stream));
+  });
+}
+
+int wk_dvals(wk_engine *E, const double *dy_dev, int64_t lddy, const double *x_dev, int64_t ldx,
+             int32_t k, double *dvals_dev, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (!E->has_adj) fail(WK_ERR_STATE, "engine was created without WK_ENGINE_ADJOINT");
+    if (k < 1 || lddy < k || ldx < k) fail(WK_ERR_ARG, "bad k or leading dimension");
+    if (E->nnz && (!dy_dev || !x_dev || !dvals_dev)) fail(WK_ERR_ARG, "NULL vector");
+    launch_dvals(E->d_rowptr, E->d_colind, E->nrows, E->nnz, dy_dev, lddy, x_dev, ldx, k,
+                 dvals_dev, static_cast<hipStream_t>(stream));
+  });
+}
+
+int wk_time_run(wk_engine *E, const wk_launch *cfg, int32_t which, const void *x_dev,
+                double *y_dev, int32_t k, int64_t ldx, int64_t ldy, void *stream, float *seconds) {
+  return guarded([&] {
+    check_engine(E);
+    if (!seconds) fail(WK_ERR_ARG, "seconds is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    int rc;
+    hip_check(hipEventRecord(e0, s), "event");
+    rc = which == 0 ? wk_run(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
+                    : wk_run_x(E, cfg, x_dev, y_dev, k, ldx, ldy, stream);
+    const hipError_t r1 = hipEventRecord(e1, s);
+    const hipError_t r2 = r1 == hipSuccess ? hipEventSynchronize(e1) : r1;
+    float ms = 0;
+    const hipError_t r3 = r2 == hipSuccess ? hipEventElapsedTime(&ms, e0, e1) : r2;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != WK_OK) fail(rc, g_last_error);
+    hip_check(r3, "timing events");
+    *seconds = ms * 1e-3f;
+  });
+}
+
+}  // extern "C"

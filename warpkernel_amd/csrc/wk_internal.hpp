@@ -1,0 +1,129 @@
+// wk_internal.hpp -- types shared by the host scan, the C ABI and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/warpkernel_c.h"
+
+// Host-side WPK layout: the result of structure::scan / structure2::scan
+// (warpkernel.hpp:276-286, 561-565), generic in the group width W.
+struct wk_layout {
+  int32_t variant = 1;
+  int32_t W = 64;
+  int32_t threshold = 0;
+  int32_t max_tpr = 32;
+  int32_t nrows = 0, ncols = 0, nznrows = 0, nwarps = 0;
+  int64_t nz = 0, allocate_nz = 0;
+  uint32_t min_nz = 0, max_nz = 0;
+  std::vector<int32_t> row_map, row_map_inv;
+  std::vector<uint32_t> reorder_rows;
+  std::vector<int64_t> warp_offsets;
+  std::vector<uint32_t> max_nz_rows;
+  std::vector<uint32_t> threads_per_row;  // variant 2
+  std::vector<uint32_t> row_offset_warp;  // variant 2
+};
+
+namespace wk {
+
+// Error plumbing: every C entry point catches wk::Error and returns its code.
+struct Error {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+void hip_check(hipError_t e, const char *what);
+void set_last_error(const std::string &msg);
+
+void validate_csr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowptr,
+                  const int32_t *colind);
+void scan1(wk_layout &L, int32_t W, int32_t nrows, int32_t ncols, int64_t nnz,
+           const int32_t *rowptr);
+void scan2(wk_layout &L, int32_t W, int32_t threshold, int32_t max_tpr, int32_t nrows,
+           int32_t ncols, int64_t nnz, const int32_t *rowptr);
+// Host scatter through reorder_rows into zero-filled padded arrays (T = double or float).
+template <class T>
+void process_host(const wk_layout &L, const double *vals, const int32_t *colind, T *pvals,
+                  int32_t *pcols);
+void reorder_columns_coalesced(const wk_layout &L, const int32_t *colind, int32_t *out);
+// CSR of A^T with the entries of each column in increasing row order;
+// perm[t] = CSR index of transposed entry t.
+void transpose_csr(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *colind,
+                   std::vector<int32_t> &colptr, std::vector<int32_t> &rowind,
+                   std::vector<int64_t> &perm);
+// Run fn(begin, end) over [0, n) on up to 16 host threads.
+void parallel_for(int64_t n, int64_t min_chunk, const std::function<void(int64_t, int64_t)> &fn);
+
+// One WPK layout resident on the device.  All pointers are device memory owned
+// by the engine.
+struct DevLayout {
+  int32_t variant = 1, W = 64;
+  int32_t nrows = 0;      // rows of the product (ncols of A for the transpose)
+  int32_t nznrows = 0, ngroups = 0;
+  int64_t allocate_nz = 0;
+  void *vals = nullptr;           // [allocate_nz] double or float
+  int32_t *cols = nullptr;        // [allocate_nz]
+  int32_t *cols_kr = nullptr;     // [allocate_nz] Kr columns (optional)
+  int32_t *row_map = nullptr;     // [nrows]
+  int64_t *warp_offsets = nullptr;   // [ngroups+1]
+  uint32_t *max_nz_rows = nullptr;   // [ngroups]
+  uint32_t *tpr = nullptr;           // [ngroups] (variant 2)
+  uint32_t *row_off = nullptr;       // [ngroups+1] (variant 2)
+  uint32_t *slot_of_nz = nullptr;    // [nnz] CSR index -> padded slot (GPU process)
+};
+
+}  // namespace wk
+
+struct wk_engine {
+  int32_t dtype = WK_F64;
+  int32_t nrows = 0, ncols = 0;
+  int64_t nnz = 0;
+  wk_layout_info info{};
+  wk::DevLayout fwd;
+  bool has_adj = false;
+  wk::DevLayout adj;                // layout of A^T
+  int32_t *d_rowptr = nullptr;      // CSR pattern for dvals (adjoint engines)
+  int32_t *d_colind = nullptr;
+  std::vector<void *> owned;        // every hipMalloc of this engine
+  int device = 0;
+};
+
+namespace wk {
+
+// Kernel-side argument block of one SpMV launch.
+struct SpmvArgs {
+  const void *vals;
+  const int32_t *cols;
+  const int32_t *row_map;   // nullptr: store in sorted order (run_x)
+  const int64_t *warp_offsets;
+  const uint32_t *max_nz_rows;
+  const uint32_t *tpr;      // variant 2
+  const uint32_t *row_off;  // variant 2
+  const void *x;
+  double *y;
+  const double *b;          // residual: y = b - A x
+  int64_t ldx, ldy, ldb;
+  int32_t ngroups, nznrows, nrows, k;
+};
+
+enum Out { OUT_RUN = 0, OUT_RUNX = 1, OUT_RESID = 2 };
+
+// Launches one SpMV over a device layout.  prec: WK_F64, or WK_F32 with
+// cfg.accumulate.  Throws wk::Error on unsupported options.
+void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int out,
+                 const void *x, int64_t ldx, double *y, int64_t ldy, const double *b,
+                 int64_t ldb, int32_t k, hipStream_t stream);
+
+// Zero-fill then scatter CSR values into the padded slots (GPU process).
+void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
+                           hipStream_t stream);
+void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
+void launch_dvals(const int32_t *rowptr, const int32_t *colind, int32_t nrows, int64_t nnz,
+                  const double *dy, int64_t lddy, const double *x, int64_t ldx, int32_t k,
+                  double *dvals, hipStream_t stream);
+
+}  // namespace wk
