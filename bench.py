@@ -126,9 +126,13 @@ def extras(torch, dev, steps, warmup):
         dx64, dx32 = dev(x), dev(xq)
         y64 = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
         y32 = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
+        e64.tune(dx64, y64, reps=5)
+        e32.tune(dx32, y32, reps=5)
         _, ms64 = time_launches(torch, lambda: e64.launch(dx64, y64), steps, warmup)
         _, ms32 = time_launches(torch, lambda: e32.launch(dx32, y32), steps, warmup)
+        e32.tune(dx32, y32, reps=5, base=dict(accumulate=1))
         _, ms32d = time_launches(torch, lambda: e32.launch(dx32, y32, cfg=dict(accumulate=1)), steps, warmup)
+        e32.tune(dx32, y32, reps=5, base=dict(accumulate=0))
         e32.launch(dx32, y32)
         torch.cuda.synchronize()
         ref_q = O.csr_spmv(m.rowptr, m.colind, vq.astype(np.float64), xq.astype(np.float64))
@@ -163,6 +167,8 @@ def extras(torch, dev, steps, warmup):
         y = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
         dxv = torch.empty((m.ncols, k), dtype=torch.float64, device="cuda")
         dv = torch.empty(m.nnz, dtype=torch.float64, device="cuda")
+        eng.tune(x, y, reps=5)
+        eng.tune(dyv, dxv, op=2, reps=5)
         _, f = time_launches(torch, lambda: eng.launch(x, y), steps, warmup)
         _, a = time_launches(torch, lambda: eng.run_adjoint(dyv, dxv), steps, warmup)
         _, d = time_launches(torch, lambda: eng.dvals(dyv, x, dv), steps, warmup)
@@ -182,6 +188,7 @@ def main():
     ap.add_argument("--threshold", type=int, default=27)
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (layout default)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration)")
@@ -207,6 +214,10 @@ def main():
     eng = engine(s, m.vals, m.colind, kr=not args.headline_only)
     dx = dev(x)
     dy = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+    tuned = None
+    if not args.no_tune:
+        print("[bench] autotuning launch shape", file=sys.stderr, flush=True)
+        tuned, tuned_ms = eng.tune(dx, dy, reps=10)
     print("[bench] timing", file=sys.stderr, flush=True)
     wall, ev_ms = time_launches(torch, lambda: eng.launch(dx, dy), args.steps, args.warmup)
     nbytes = spmv_bytes(m.nnz, m.nrows, m.ncols)
@@ -236,7 +247,8 @@ def main():
                         f"{m.nrows} rows, {m.nnz} nnz, E~lognormal(0,0.5), nu=0.3, x~U[-1,1]",
             "kernel": f"{args.variant.upper()} engine::run, group width 64, k=1, FP64 values, int32 indices",
             "threshold": args.threshold if args.variant == "wpk2" else None,
-            "launch": "default (4 waves/block, unroll 8, XCD remap, nontemporal loads)",
+            "launch": eng.get_launch().as_dict(),
+            "launch_source": "autotuner (wk_tune)" if tuned else "layout default",
             "alg_bytes_per_step": nbytes,
             "gflops": round(2 * m.nnz / (wall / args.steps) / 1e9, 1),
             "result_bitexact_vs_oracle": ok,
@@ -249,11 +261,13 @@ def main():
     if not args.headline_only:
         dxr = dev(s.reorder_x(x))
         dyr = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+        eng.tune(dxr, dyr, op=1, reps=10)
         _, msx = time_launches(torch, lambda: eng.launch_x(dxr, dyr), args.steps, args.warmup)
         ex = {"run_x_GBs": round(nbytes / (msx * 1e-3) / 1e9, 2), "run_x_ms": round(msx, 4)}
         del eng
         s2 = structure2(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, args.threshold)
         e2 = engine(s2, m.vals, m.colind)
+        e2.tune(dx, dy, reps=10)
         _, ms2 = time_launches(torch, lambda: e2.launch(dx, dy), args.steps, args.warmup)
         ex[f"wpk2_t{args.threshold}_GBs"] = round(nbytes / (ms2 * 1e-3) / 1e9, 2)
         del e2

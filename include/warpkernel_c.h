@@ -156,6 +156,11 @@ typedef struct wk_launch {
   int32_t xcd_remap;       /* 1: contiguous group ranges per XCD */
   int32_t nontemporal;     /* 1: streaming loads for values and indices */
   int32_t accumulate;      /* WK_ACC_* for FP32 engines */
+  int32_t rows_per_lane;   /* 1, 2 or 4 groups per wave (k_tile 1, group width 64) */
+  int32_t blocks_per_cu;   /* 0: no cap; else resident blocks per CU are capped
+                              by reserving LDS (occupancy control) */
+  int32_t stage_lds;       /* 1: copy value/index batches through LDS with
+                              16-byte loads (k_tile 1, group width 64, unroll 8/16) */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);
@@ -190,6 +195,31 @@ int wk_run_adjoint(wk_engine *engine, const wk_launch *cfg, const void *dy_dev, 
  * counterpart. */
 int wk_dvals(wk_engine *engine, const double *dy_dev, int64_t lddy, const double *x_dev,
              int64_t ldx, int32_t k, double *dvals_dev, void *stream);
+
+/* ---------------------------------------------------------------- tuning */
+
+/* Operations an engine launches; the engine keeps one launch configuration
+ * per (operation, k), used whenever cfg is NULL. */
+#define WK_OP_RUN 0
+#define WK_OP_RUN_X 1
+#define WK_OP_ADJOINT 2
+
+/* Autotuner (no reference counterpart; the reference sweeps warps/block and
+ * the WPK2 threshold in its drivers, wpk1.cu:138-257, wpk2.cu:156-302).  Times
+ * every candidate launch configuration of operation `op` on this engine with
+ * the given device vectors (k right-hand sides, reps launches each, HIP
+ * events on `stream`), stores the fastest in the engine for (op, k) and
+ * returns it with its mean time.  Only performance knobs vary; base (may be
+ * NULL) fixes the numerics (accumulate).  Results are identical for every
+ * candidate.  y_dev is overwritten. */
+int wk_tune(wk_engine *engine, int32_t op, const wk_launch *base, const void *x_dev,
+            double *y_dev, int32_t k, int64_t ldx, int64_t ldy, int32_t reps, void *stream,
+            wk_launch *best, float *best_ms);
+
+/* Read / replace the configuration the engine uses for (op, k) when cfg is
+ * NULL (before tuning: a choice from the layout's mean row length). */
+int wk_engine_get_launch(const wk_engine *engine, int32_t op, int32_t k, wk_launch *cfg);
+int wk_engine_set_launch(wk_engine *engine, int32_t op, int32_t k, const wk_launch *cfg);
 
 /* Blocking timing helper with the reference's contract (engine::run returns
  * the seconds of one launch, warpkernel.hpp:829-843): launches once, waits,

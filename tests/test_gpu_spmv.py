@@ -137,18 +137,32 @@ def test_dropin_driver_flow_wpk1():
 
 @pytest.mark.parametrize("cfg", [
     dict(waves_per_block=1), dict(waves_per_block=8), dict(unroll=4), dict(unroll=16),
-    dict(xcd_remap=0), dict(nontemporal=0),
+    dict(xcd_remap=0), dict(nontemporal=0), dict(rows_per_lane=2), dict(rows_per_lane=2, unroll=4),
+    dict(rows_per_lane=4, unroll=4), dict(rows_per_lane=4), dict(blocks_per_cu=3),
+    dict(rows_per_lane=2, waves_per_block=1, blocks_per_cu=6),
+    dict(stage_lds=1), dict(stage_lds=1, unroll=16), dict(stage_lds=1, waves_per_block=1, unroll=16),
 ])
 def test_launch_options_do_not_change_results(cfg):
     say(f"[gpu] launch options {cfg}")
-    m = matrix("hex_elastic_3d_32")
+    m = fe.tet_poisson_3d(40, drop=0.05)  # mixed row lengths, empty rows, odd group count
     x = fe.uniform_x(m.ncols, seed=2004)
     ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
     s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
-    eng = engine(s, m.vals, m.colind)
-    dy = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+    eng = engine(s, m.vals, m.colind, kr=True)
+    dy = torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda")
     eng.launch(dev(x), dy, cfg=cfg)
     assert np.array_equal(host(dy), ref)
+    dyr = torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    eng.launch_x(dev(s.reorder_x(x)), dyr, cfg=cfg)
+    assert np.array_equal(host(dyr), ref[s.row_map])
+    # WPK2: split groups next to unsplit ones in one wave
+    L = O.scan2(m.rowptr, 4, 64, 32)
+    pv, pc = O.process(L, m.vals, m.colind)
+    s2 = structure2().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, 4)
+    e2 = engine(s2, m.vals, m.colind)
+    dy2 = torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    e2.launch(dev(x), dy2, cfg=cfg)
+    assert np.array_equal(host(dy2), O.wpk_spmv(L, pv, pc, x))
 
 
 def test_empty_rows_written_as_zero():
@@ -158,13 +172,16 @@ def test_empty_rows_written_as_zero():
     assert empty.any()
     x = fe.uniform_x(m.ncols, seed=2005)
     ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
-    for cls, args in ((structure, ()), (structure2, (3,))):
+    L2 = O.scan2(m.rowptr, 3, 64, 32)
+    pv, pc = O.process(L2, m.vals, m.colind)
+    ref2 = O.wpk_spmv(L2, pv, pc, x)  # tree order of the split rows
+    for cls, args, want in ((structure, (), ref), (structure2, (3,), ref2)):
         s = cls().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, *args)
         eng = engine(s, m.vals, m.colind, kr=True)
         dy = torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda")
         eng.launch(dev(x), dy)
         y = host(dy)
-        assert np.all(y[empty] == 0) and np.array_equal(y[~empty], ref[~empty])
+        assert np.all(y[empty] == 0) and np.array_equal(y, want)
 
 
 def test_tiny_and_empty_matrices():
@@ -235,3 +252,28 @@ def test_errors_cross_the_abi():
         eng.run_adjoint(x, r)  # no transposed layout
     with pytest.raises(WKError):
         eng.launch(x, r, cfg=dict(waves_per_block=3))
+
+
+def test_autotuner_keeps_results_and_stores_choice():
+    say("[gpu] autotuner")
+    m = fe.tet_poisson_3d(40, drop=0.05)
+    x = fe.uniform_x(m.ncols, seed=2008)
+    ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind, kr=True, adjoint=True)
+    dx, dy = dev(x), torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+    best, ms = eng.tune(dx, dy, reps=3)
+    assert ms > 0 and eng.get_launch().as_dict() == best
+    dy.fill_(float("nan"))
+    eng.launch(dx, dy)
+    assert np.array_equal(host(dy), ref)
+    best_adj, _ = eng.tune(dy, torch.empty(m.ncols, dtype=torch.float64, device="cuda"), op=2, reps=3)
+    assert eng.get_launch(op=2).as_dict() == best_adj
+    k = 8
+    xk = fe.uniform_x(m.ncols, k, seed=2009).reshape(m.ncols, k)
+    yk = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
+    bk, _ = eng.tune(dev(xk), yk, reps=3)
+    assert bk["k_tile"] in (1, 2, 4, 8)
+    yk.fill_(float("nan"))
+    eng.launch(dev(xk), yk)
+    assert np.array_equal(host(yk), O.csr_spmv(m.rowptr, m.colind, m.vals, xk))

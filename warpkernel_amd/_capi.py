@@ -41,6 +41,7 @@ class Launch(C.Structure):
     _fields_ = [
         ("waves_per_block", C.c_int32), ("unroll", C.c_int32), ("k_tile", C.c_int32),
         ("xcd_remap", C.c_int32), ("nontemporal", C.c_int32), ("accumulate", C.c_int32),
+        ("rows_per_lane", C.c_int32), ("blocks_per_cu", C.c_int32), ("stage_lds", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -76,7 +77,12 @@ SIGNATURES = {
     "wk_run_adjoint": (C.c_int, [P, C.POINTER(Launch), P, P, I32, I64, I64, P]),
     "wk_dvals": (C.c_int, [P, P, I64, P, I64, I32, P, P]),
     "wk_time_run": (C.c_int, [P, C.POINTER(Launch), I32, P, P, I32, I64, I64, P, C.POINTER(C.c_float)]),
+    "wk_tune": (C.c_int, [P, I32, C.POINTER(Launch), P, P, I32, I64, I64, I32, P, C.POINTER(Launch),
+                          C.POINTER(C.c_float)]),
+    "wk_engine_get_launch": (C.c_int, [P, I32, I32, C.POINTER(Launch)]),
+    "wk_engine_set_launch": (C.c_int, [P, I32, I32, C.POINTER(Launch)]),
 }
+WK_OP_RUN, WK_OP_RUN_X, WK_OP_ADJOINT = 0, 1, 2
 
 _lib = None
 

@@ -189,32 +189,68 @@ class engine:
         self.nrows, self.ncols, self.nz = k.nrows, k.ncols, k.nz
         self.has_kr, self.has_adjoint = kr, adjoint
 
+    # -- launch configuration ---------------------------------------------
+    def get_launch(self, op: int = capi.WK_OP_RUN, k: int = 1) -> capi.Launch:
+        """The configuration used for (op, k) when no cfg is passed."""
+        c = capi.Launch()
+        check(lib().wk_engine_get_launch(self._h, op, k, C.byref(c)))
+        return c
+
+    def set_launch(self, cfg, op: int = capi.WK_OP_RUN, k: int = 1):
+        c = cfg if isinstance(cfg, capi.Launch) else self._cfg(op, k, cfg)
+        check(lib().wk_engine_set_launch(self._h, op, k, C.byref(c)))
+
+    def _cfg(self, op, k, cfg):
+        """None -> NULL (engine's own choice); dict -> overrides on top of it."""
+        if cfg is None:
+            return None
+        if isinstance(cfg, capi.Launch):
+            return cfg
+        c = self.get_launch(op, k)
+        for key, v in cfg.items():
+            if not hasattr(c, key):
+                raise KeyError(key)
+            setattr(c, key, int(v))
+        return c
+
+    @staticmethod
+    def _ref(c):
+        return None if c is None else C.byref(c)
+
+    def tune(self, x, y, op: int = capi.WK_OP_RUN, reps: int = 10, base=None, stream=None):
+        """Time every candidate launch shape for (op, k = columns of x) and keep
+        the fastest in the engine.  Returns (config dict, mean ms)."""
+        k, ldx = self._k_ld(x)
+        _, ldy = self._k_ld(y)
+        best = capi.Launch()
+        ms = C.c_float()
+        b = self._cfg(op, k, base)
+        check(lib().wk_tune(self._h, op, self._ref(b), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, reps,
+                            _stream(stream), C.byref(best), C.byref(ms)))
+        return best.as_dict(), ms.value
+
     # -- reference-shaped, blocking, returns seconds --------------------
-    def _cfg(self, blocksize, cfg):
-        c = capi.default_launch(**(cfg or {}))
+    def _timed(self, which, blocksize, dx, dy, cfg, stream):
+        op = capi.WK_OP_RUN if which == 0 else capi.WK_OP_RUN_X
+        c = self._cfg(op, 1, cfg or {})
         if blocksize:
             c.waves_per_block = max(1, min(8, int(blocksize) // 64))
-        return c
+        sec = C.c_float()
+        check(lib().wk_time_run(self._h, C.byref(c), which, _dev_ptr(dx), _dev_ptr(dy), 1, 1, 1,
+                                _stream(stream), C.byref(sec)))
+        return sec.value
 
     def run(self, nblocks, blocksize, dx, dy, cfg=None, stream=None) -> float:
         """engine::run<cache>(nblocks, blocksize, dx, dy): seconds of one launch.
         The grid is derived from the layout; nblocks is accepted for the
         reference's signature.  blocksize / 64 = waves per block (1..8)."""
         del nblocks
-        sec = C.c_float()
-        c = self._cfg(blocksize, cfg)
-        check(lib().wk_time_run(self._h, C.byref(c), 0, _dev_ptr(dx), _dev_ptr(dy), 1, 1, 1,
-                                _stream(stream), C.byref(sec)))
-        return sec.value
+        return self._timed(0, blocksize, dx, dy, cfg, stream)
 
     def run_x(self, nblocks, blocksize, dx, dy, cfg=None, stream=None) -> float:
         """engine::run_x<cache>: the Kr form, y'[i] for x' = reorder_x(x)."""
         del nblocks
-        sec = C.c_float()
-        c = self._cfg(blocksize, cfg)
-        check(lib().wk_time_run(self._h, C.byref(c), 1, _dev_ptr(dx), _dev_ptr(dy), 1, 1, 1,
-                                _stream(stream), C.byref(sec)))
-        return sec.value
+        return self._timed(1, blocksize, dx, dy, cfg, stream)
 
     # -- asynchronous forms ---------------------------------------------
     @staticmethod
@@ -229,30 +265,30 @@ class engine:
         """y = A x (k right-hand sides if x, y are [n, k] row-major)."""
         k, ldx = self._k_ld(x)
         _, ldy = self._k_ld(y)
-        c = capi.default_launch(**(cfg or {}))
-        check(lib().wk_run(self._h, C.byref(c), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, _stream(stream)))
+        c = self._cfg(capi.WK_OP_RUN, k, cfg)
+        check(lib().wk_run(self._h, self._ref(c), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, _stream(stream)))
 
     def launch_x(self, x, y, cfg=None, stream=None):
         k, ldx = self._k_ld(x)
         _, ldy = self._k_ld(y)
-        c = capi.default_launch(**(cfg or {}))
-        check(lib().wk_run_x(self._h, C.byref(c), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, _stream(stream)))
+        c = self._cfg(capi.WK_OP_RUN_X, k, cfg)
+        check(lib().wk_run_x(self._h, self._ref(c), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, _stream(stream)))
 
     def residual(self, b, x, r, cfg=None, stream=None):
         """r = b - A x in FP64."""
         k, ldx = self._k_ld(x)
         _, ldb = self._k_ld(b)
         _, ldr = self._k_ld(r)
-        c = capi.default_launch(**(cfg or {}))
-        check(lib().wk_residual(self._h, C.byref(c), _dev_ptr(b), ldb, _dev_ptr(x), _dev_ptr(r), k,
+        c = self._cfg(capi.WK_OP_RUN, k, cfg)
+        check(lib().wk_residual(self._h, self._ref(c), _dev_ptr(b), ldb, _dev_ptr(x), _dev_ptr(r), k,
                                 ldx, ldr, _stream(stream)))
 
     def run_adjoint(self, dy, dx, cfg=None, stream=None):
         """dx = A^T dy through the transposed layout."""
         k, lddy = self._k_ld(dy)
         _, lddx = self._k_ld(dx)
-        c = capi.default_launch(**(cfg or {}))
-        check(lib().wk_run_adjoint(self._h, C.byref(c), _dev_ptr(dy), _dev_ptr(dx), k, lddy, lddx,
+        c = self._cfg(capi.WK_OP_ADJOINT, k, cfg)
+        check(lib().wk_run_adjoint(self._h, self._ref(c), _dev_ptr(dy), _dev_ptr(dx), k, lddy, lddx,
                                    _stream(stream)))
 
     def dvals(self, dy, x, out, stream=None):

@@ -111,11 +111,8 @@ void upload_layout(wk_engine *E, DevLayout &D, const wk_layout &L, const double 
   hip_check(hipStreamSynchronize(s), "upload");
 }
 
-wk_launch resolve(const wk_launch *cfg) {
-  wk_launch c;
-  wk_launch_default(&c);
-  if (cfg) c = *cfg;
-  return c;
+wk_launch resolve(wk_engine *E, int32_t op, int32_t k, const wk_launch *cfg) {
+  return cfg ? *cfg : engine_launch(E, op, k);
 }
 
 void check_engine(const wk_engine *E) {
@@ -336,6 +333,9 @@ void wk_launch_default(wk_launch *cfg) {
   cfg->xcd_remap = 1;
   cfg->nontemporal = 1;
   cfg->accumulate = WK_ACC_TWOSUM;
+  cfg->rows_per_lane = 1;
+  cfg->blocks_per_cu = 0;
+  cfg->stage_lds = 0;
 }
 
 int wk_run(wk_engine *E, const wk_launch *cfg, const void *x_dev, double *y_dev, int32_t k,
@@ -344,7 +344,7 @@ int wk_run(wk_engine *E, const wk_launch *cfg, const void *x_dev, double *y_dev,
     check_engine(E);
     if ((!x_dev && E->ncols) || (!y_dev && E->nrows)) fail(WK_ERR_ARG, "NULL vector");
     if (ldx < k || ldy < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
-    launch_spmv(E->fwd, E->dtype, resolve(cfg), OUT_RUN, x_dev, ldx, y_dev, ldy, nullptr, 0, k,
+    launch_spmv(E->fwd, E->dtype, resolve(E, WK_OP_RUN, k, cfg), OUT_RUN, x_dev, ldx, y_dev, ldy, nullptr, 0, k,
                 static_cast<hipStream_t>(stream));
   });
 }
@@ -356,7 +356,7 @@ int wk_run_x(wk_engine *E, const wk_launch *cfg, const void *x_dev, double *y_de
     if (E->nrows != E->ncols) fail(WK_ERR_ARG, "the Kr form needs a square matrix");
     if ((!x_dev && E->ncols) || (!y_dev && E->nrows)) fail(WK_ERR_ARG, "NULL vector");
     if (ldx < k || ldy < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
-    launch_spmv(E->fwd, E->dtype, resolve(cfg), OUT_RUNX, x_dev, ldx, y_dev, ldy, nullptr, 0, k,
+    launch_spmv(E->fwd, E->dtype, resolve(E, WK_OP_RUN_X, k, cfg), OUT_RUNX, x_dev, ldx, y_dev, ldy, nullptr, 0, k,
                 static_cast<hipStream_t>(stream));
   });
 }
@@ -369,7 +369,7 @@ int wk_residual(wk_engine *E, const wk_launch *cfg, const double *b_dev, int64_t
     if (E->dtype != WK_F64) fail(WK_ERR_ARG, "the residual is computed by FP64 engines");
     if ((!x_dev && E->ncols) || ((!r_dev || !b_dev) && E->nrows)) fail(WK_ERR_ARG, "NULL vector");
     if (ldx < k || ldr < k || ldb < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
-    launch_spmv(E->fwd, E->dtype, resolve(cfg), OUT_RESID, x_dev, ldx, r_dev, ldr, b_dev, ldb, k,
+    launch_spmv(E->fwd, E->dtype, resolve(E, WK_OP_RUN, k, cfg), OUT_RESID, x_dev, ldx, r_dev, ldr, b_dev, ldb, k,
                 static_cast<hipStream_t>(stream));
   });
 }
@@ -381,7 +381,7 @@ int wk_run_adjoint(wk_engine *E, const wk_launch *cfg, const void *dy_dev, doubl
     if (!E->has_adj) fail(WK_ERR_STATE, "engine was created without WK_ENGINE_ADJOINT");
     if ((!dy_dev && E->nrows) || (!dx_dev && E->ncols)) fail(WK_ERR_ARG, "NULL vector");
     if (lddy < k || lddx < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
-    launch_spmv(E->adj, E->dtype, resolve(cfg), OUT_RUN, dy_dev, lddy, dx_dev, lddx, nullptr, 0,
+    launch_spmv(E->adj, E->dtype, resolve(E, WK_OP_ADJOINT, k, cfg), OUT_RUN, dy_dev, lddy, dx_dev, lddx, nullptr, 0,
                 k, static_cast<hipStream_t>(stream));
   });
 }

@@ -126,15 +126,79 @@ __device__ __forceinline__ int32_t work_id(int32_t b, int32_t nwork, int32_t rem
 }
 
 // ----------------------------------------------------------------- kernel
+// One group's slice of a wave: base offset, steps, lanes per row, first row.
+struct GroupMeta {
+  int64_t base;
+  uint32_t len, tpr, row0;
+  bool valid;
+};
+
+template <int W, bool SPLIT>
+__device__ __forceinline__ GroupMeta load_meta(const SpmvArgs &a, int32_t g) {
+  GroupMeta m;
+  m.valid = g < a.ngroups;
+  m.base = m.valid ? a.warp_offsets[g] : 0;
+  m.len = m.valid ? a.max_nz_rows[g] : 0u;
+  m.tpr = SPLIT && m.valid ? a.tpr[g] : 1u;
+  m.row0 = SPLIT ? (m.valid ? a.row_off[g] : 0u) : (uint32_t)g * (uint32_t)W;
+  return m;
+}
+
+// Steps [i, end) of R groups in lockstep, U slots per group per batch.  All
+// loads of a batch are issued before the first gather waits, so a wave has
+// R*U value+index loads in flight per batch.  Values stream with the
+// nontemporal hint (NT); 4-byte index loads measured faster without it.
+template <class P, int W, int KT, int U, bool NT, int R>
+__device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
+                                                 const typename P::V *const (&vp)[R],
+                                                 const int32_t *const (&cp)[R],
+                                                 const typename P::X *xb, int64_t ldx,
+                                                 uint32_t i, uint32_t end) {
+  using V = typename P::V;
+  using X = typename P::X;
+  for (; i + U <= end; i += U) {
+    V v[R][U];
+    int32_t c[R][U];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        c[r][u] = cp[r][(int64_t)(i + u) * W];
+        v[r][u] = ld_stream<NT>(vp[r] + (int64_t)(i + u) * W);
+      }
+    X xv[R][U][KT];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const X *xr = xb + (int64_t)c[r][u] * ldx;
+#pragma unroll
+        for (int j = 0; j < KT; j++) xv[r][u][j] = xr[j];
+      }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < KT; j++) P::step(acc[r][j], v[r][u], xv[r][u][j]);
+  }
+  return i;
+}
+
 // grid.x = nwork + fill blocks.  Work block -> (group block, k-tile), k-tile
-// fastest, so blocks running together stream the same groups.  Fill blocks
-// write the rows without nonzeros (sorted positions nznrows..nrows-1).
-template <class P, int W, bool SPLIT, int OUT, int KT, int U, bool NT>
+// fastest, so blocks running together stream the same groups.  With W = 64 a
+// wave owns R consecutive groups ("rows per lane" = R); with W = 32 it owns
+// two groups, one per half-wave.  Fill blocks write the rows without nonzeros
+// (sorted positions nznrows..nrows-1).
+template <class P, int W, bool SPLIT, int OUT, int KT, int U, bool NT, int R>
 __global__ __launch_bounds__(kMaxThreads) void spmv_kernel(SpmvArgs a, int32_t nwork,
                                                            int32_t ktiles, int32_t remap) {
   using V = typename P::V;
   using X = typename P::X;
   using Acc = typename P::Acc;
+  static_assert(W == 64 || R == 1, "half-wave groups take one group per half");
+  extern __shared__ char occupancy_cap[];  // sized by the launcher only to limit blocks per CU
+  (void)occupancy_cap;
   const int32_t lane = threadIdx.x & 63;
   const int32_t wpb = blockDim.x >> 6;
   if ((int32_t)blockIdx.x >= nwork) {
@@ -150,90 +214,189 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_kernel(SpmvArgs a, int32_t n
   const int32_t j0 = kt * KT;
   const int32_t wave = __builtin_amdgcn_readfirstlane(gb * wpb + (int32_t)(threadIdx.x >> 6));
 
-  int32_t g, l;
+  int32_t l;
+  GroupMeta m[R];
   if constexpr (W == 64) {
-    g = wave;
     l = lane;
-    if (g >= a.ngroups) return;
+    if (wave * R >= a.ngroups) return;
+#pragma unroll
+    for (int r = 0; r < R; r++) m[r] = load_meta<W, SPLIT>(a, wave * R + r);
   } else {
-    g = 2 * wave + (lane >> 5);
     l = lane & 31;
     if (2 * wave >= a.ngroups) return;
+    m[0] = load_meta<W, SPLIT>(a, 2 * wave + (lane >> 5));
   }
-  const bool active = g < a.ngroups;
-  const int64_t base = active ? a.warp_offsets[g] : 0;
-  const uint32_t len = active ? a.max_nz_rows[g] : 0u;
-  const uint32_t tpr = SPLIT && active ? a.tpr[g] : 1u;
-  const uint32_t row0 = SPLIT ? (active ? a.row_off[g] : 0u) : (uint32_t)g * (uint32_t)W;
 
-  const V *__restrict__ vp = static_cast<const V *>(a.vals) + base + l;
-  const int32_t *__restrict__ cp = a.cols + base + l;
+  const V *vp[R];
+  const int32_t *cp[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    vp[r] = static_cast<const V *>(a.vals) + m[r].base + l;
+    cp[r] = a.cols + m[r].base + l;
+  }
+  const V *const(&vpc)[R] = vp;
+  const int32_t *const(&cpc)[R] = cp;
   const X *__restrict__ xb = static_cast<const X *>(a.x) + j0;
   const int64_t ldx = a.ldx;
 
-  Acc acc[KT];
-  if (len > 0) {
-    const V v = ld_stream<NT>(vp);
-    const int32_t c = ld_stream<NT>(cp);
-    const X *xr = xb + (int64_t)c * ldx;
+  // first slot of each row: rounded product (warpkernel.hpp:68)
+  Acc acc[R][KT];
+  uint32_t common = 0xffffffffu;
 #pragma unroll
-    for (int j = 0; j < KT; j++) P::first(acc[j], v, xr[j]);
-  } else {
+  for (int r = 0; r < R; r++) {
+    if (m[r].len > 0) {
+      const int32_t c = *cp[r];
+      const V v = ld_stream<NT>(vp[r]);
+      const X *xr = xb + (int64_t)c * ldx;
 #pragma unroll
-    for (int j = 0; j < KT; j++) P::first(acc[j], V(0), X(0));
-  }
-  uint32_t i = 1;
-  for (; i + U <= len; i += U) {
-    V v[U];
-    int32_t c[U];
+      for (int j = 0; j < KT; j++) P::first(acc[r][j], v, xr[j]);
+    } else {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      v[u] = ld_stream<NT>(vp + (int64_t)(i + u) * W);
-      c[u] = ld_stream<NT>(cp + (int64_t)(i + u) * W);
+      for (int j = 0; j < KT; j++) P::first(acc[r][j], V(0), X(0));
     }
-    X xv[U][KT];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const X *xr = xb + (int64_t)c[u] * ldx;
-#pragma unroll
-      for (int j = 0; j < KT; j++) xv[u][j] = xr[j];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-      for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
+    common = m[r].len < common ? m[r].len : common;
   }
-  for (; i < len; i++) {
-    const V v = ld_stream<NT>(vp + (int64_t)i * W);
-    const int32_t c = ld_stream<NT>(cp + (int64_t)i * W);
-    const X *xr = xb + (int64_t)c * ldx;
+  // all R groups together up to the shortest, then each group alone
+  uint32_t i = stream_steps<P, W, KT, U, NT, R>(acc, vpc, cpc, xb, ldx, 1u, common);
 #pragma unroll
-    for (int j = 0; j < KT; j++) P::step(acc[j], v, xr[j]);
+  for (int r = 0; r < R; r++) {
+    const V *const vr[1] = {vp[r]};
+    const int32_t *const cr[1] = {cp[r]};
+    Acc ar[1][KT];
+#pragma unroll
+    for (int j = 0; j < KT; j++) ar[0][j] = acc[r][j];
+    uint32_t ii = stream_steps<P, W, KT, U, NT, 1>(ar, vr, cr, xb, ldx, i, m[r].len);
+#pragma unroll
+    for (int j = 0; j < KT; j++) acc[r][j] = ar[0][j];
+    for (; ii < m[r].len; ii++) {
+      const int32_t c = cp[r][(int64_t)ii * W];
+      const V v = ld_stream<NT>(vp[r] + (int64_t)ii * W);
+      const X *xr = xb + (int64_t)c * ldx;
+#pragma unroll
+      for (int j = 0; j < KT; j++) P::step(acc[r][j], v, xr[j]);
+    }
   }
 
-  if constexpr (SPLIT) {
-    // lockstep pairwise tree of hpp:157-162: lane t adds lane t+o's partial
-    for (uint32_t o = 1; o < tpr; o <<= 1) {
 #pragma unroll
-      for (int j = 0; j < KT; j++) {
-        Acc other = P::shfl_down(acc[j], (int)o, W);
-        P::add(acc[j], other);
+  for (int r = 0; r < R; r++) {
+    if constexpr (SPLIT) {
+      // lockstep pairwise tree of hpp:157-162: lane t adds lane t+o's partial
+      for (uint32_t o = 1; o < m[r].tpr; o <<= 1) {
+#pragma unroll
+        for (int j = 0; j < KT; j++) {
+          Acc other = P::shfl_down(acc[r][j], (int)o, W);
+          P::add(acc[r][j], other);
+        }
       }
     }
-  }
-  if (!active) return;
-  const uint32_t rowid = row0 + (uint32_t)l / tpr;
-  if (((uint32_t)l & (tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
-  const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
-  double *yr = a.y + dst * a.ldy + j0;
+    if (!m[r].valid) continue;
+    const uint32_t tpr = m[r].tpr;
+    const uint32_t rowid = m[r].row0 + (uint32_t)l / tpr;
+    if (((uint32_t)l & (tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) continue;
+    const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
+    double *yr = a.y + dst * a.ldy + j0;
 #pragma unroll
-  for (int j = 0; j < KT; j++) {
-    const double r = P::result(acc[j]);
-    if constexpr (OUT == OUT_RESID)
-      yr[j] = a.b[dst * a.ldb + j0 + j] - r;
-    else
-      yr[j] = r;
+    for (int j = 0; j < KT; j++) {
+      const double res = P::result(acc[r][j]);
+      if constexpr (OUT == OUT_RESID)
+        yr[j] = a.b[dst * a.ldb + j0 + j] - res;
+      else
+        yr[j] = res;
+    }
   }
+}
+
+// ------------------------------------------------------- LDS-staged kernel
+// Same result as spmv_kernel<P, 64, SPLIT, OUT, 1, U, true, 1>, different load
+// shape.  A batch of U steps of one group is a contiguous block of U*64 values
+// and U*64 indices; the wave copies it with 16-byte-per-lane loads (1 KB per
+// wave instruction instead of 512 B / 256 B) into its own LDS slice, and each
+// lane reads its slots back (ds_read, bank-conflict free).  Narrow per-lane
+// loads are limited by the texture-address rate, wide ones reach the HBM
+// ceiling.  Only the wave touches its slice, so no barrier is needed: LDS
+// operations of one wave complete in order.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <class P, int U>
+constexpr int staged_bytes() {
+  return U * 64 * (int)(sizeof(typename P::V) + sizeof(int32_t));
+}
+
+template <class P, bool SPLIT, int OUT, int U>
+__global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, int32_t nwork,
+                                                                  int32_t remap) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  constexpr int W = 64;
+  constexpr int NV = U * 64 * (int)sizeof(V) / 1024;  // 16-byte loads per lane, values
+  constexpr int NC = U * 64 * 4 / 1024;               // 16-byte loads per lane, indices
+  static_assert(NV >= 1 && NC >= 1 && U % 4 == 0, "a batch covers whole 1 KB chunks");
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t wpb = blockDim.x >> 6;
+  if ((int32_t)blockIdx.x >= nwork) {
+    const int64_t pos = (int64_t)a.nznrows + (int64_t)(blockIdx.x - nwork) * blockDim.x + threadIdx.x;
+    if (pos >= a.nrows) return;
+    const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
+    a.y[dst * a.ldy] = OUT == OUT_RESID ? a.b[dst * a.ldb] : 0.0;
+    return;
+  }
+  const int32_t wib = threadIdx.x >> 6;
+  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(blockIdx.x, nwork, remap) * wpb + wib);
+  if (wave >= a.ngroups) return;
+  const GroupMeta m = load_meta<W, SPLIT>(a, wave);
+  V *lv = reinterpret_cast<V *>(lds + wib * staged_bytes<P, U>());  // [U][64]
+  int32_t *lc = reinterpret_cast<int32_t *>(lds + wib * staged_bytes<P, U>() + U * 64 * sizeof(V));
+  const V *vbase = static_cast<const V *>(a.vals) + m.base;
+  const int32_t *cbase = a.cols + m.base;
+  const X *__restrict__ xb = static_cast<const X *>(a.x);
+  const int64_t ldx = a.ldx;
+
+  Acc acc;
+  {
+    const int32_t c = cbase[lane];
+    const V v = __builtin_nontemporal_load(vbase + lane);
+    P::first(acc, v, xb[(int64_t)c * ldx]);
+  }
+  uint32_t i = 1;
+  for (; i + U <= m.len; i += U) {
+    const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)i * W);
+    const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)i * W);
+    u32x4 tc[NC], tv[NV];
+#pragma unroll
+    for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+#pragma unroll
+    for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
+#pragma unroll
+    for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
+    X xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
+#pragma unroll
+    for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+  }
+  for (; i < m.len; i++) {
+    const int32_t c = cbase[(int64_t)i * W + lane];
+    const V v = __builtin_nontemporal_load(vbase + (int64_t)i * W + lane);
+    P::step(acc, v, xb[(int64_t)c * ldx]);
+  }
+  if constexpr (SPLIT) {
+    for (uint32_t o = 1; o < m.tpr; o <<= 1) {
+      Acc other = P::shfl_down(acc, (int)o, W);
+      P::add(acc, other);
+    }
+  }
+  const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
+  if (((uint32_t)lane & (m.tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
+  const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
+  const double res = P::result(acc);
+  if constexpr (OUT == OUT_RESID)
+    a.y[dst * a.ldy] = a.b[dst * a.ldb] - res;
+  else
+    a.y[dst * a.ldy] = res;
 }
 
 // ------------------------------------------------------------ small kernels
@@ -307,58 +470,106 @@ __global__ __launch_bounds__(256) void dvals_kernel(const int32_t *__restrict__ 
 }
 
 // ------------------------------------------------------------- dispatch
-using LaunchFn = void (*)(const SpmvArgs &, dim3, dim3, int32_t, int32_t, int32_t, hipStream_t);
+using LaunchFn = void (*)(const SpmvArgs &, dim3, dim3, size_t, int32_t, int32_t, int32_t,
+                          hipStream_t);
 
-template <class P, int W, bool SPLIT, int OUT, int KT, int U, bool NT>
-void launch_one(const SpmvArgs &a, dim3 grid, dim3 block, int32_t nwork, int32_t ktiles,
-                int32_t remap, hipStream_t s) {
-  hipLaunchKernelGGL((spmv_kernel<P, W, SPLIT, OUT, KT, U, NT>), grid, block, 0, s, a, nwork,
-                     ktiles, remap);
+template <class P, int W, bool SPLIT, int OUT, int KT, int U, bool NT, int R>
+void launch_one(const SpmvArgs &a, dim3 grid, dim3 block, size_t lds, int32_t nwork,
+                int32_t ktiles, int32_t remap, hipStream_t s) {
+  hipLaunchKernelGGL((spmv_kernel<P, W, SPLIT, OUT, KT, U, NT, R>), grid, block, lds, s, a,
+                     nwork, ktiles, remap);
 }
 
+// Instantiated shapes.  k_tile 1 (the bandwidth-bound case) has the unroll,
+// rows-per-lane and load-policy choices; wider k-tiles carry more registers
+// per slot and keep one shape each.
 template <class P, int W, bool SPLIT, int OUT>
-LaunchFn pick_kt(int kt, int u, bool nt) {
+LaunchFn pick_kt(int kt, int u, bool nt, int r) {
   if constexpr (W == 32) {
     // the reference's own layout width: parity path, one right-hand side per pass
-    return kt == 1 ? &launch_one<P, 32, SPLIT, OUT, 1, 8, true> : nullptr;
+    return kt == 1 && r == 1 ? &launch_one<P, 32, SPLIT, OUT, 1, 8, true, 1> : nullptr;
   } else {
+    if (kt == 1) {
+      if (!nt) return r == 1 && u == 8 ? &launch_one<P, W, SPLIT, OUT, 1, 8, false, 1> : nullptr;
+      if (r == 1) {
+        if (u == 4) return &launch_one<P, W, SPLIT, OUT, 1, 4, true, 1>;
+        if (u == 8) return &launch_one<P, W, SPLIT, OUT, 1, 8, true, 1>;
+        if (u == 16) return &launch_one<P, W, SPLIT, OUT, 1, 16, true, 1>;
+      } else if (r == 2) {
+        if (u == 4) return &launch_one<P, W, SPLIT, OUT, 1, 4, true, 2>;
+        if (u == 8) return &launch_one<P, W, SPLIT, OUT, 1, 8, true, 2>;
+      } else if (r == 4) {
+        if (u == 4) return &launch_one<P, W, SPLIT, OUT, 1, 4, true, 4>;
+        if (u == 8) return &launch_one<P, W, SPLIT, OUT, 1, 8, true, 4>;
+      }
+      return nullptr;
+    }
+    if (r != 1) return nullptr;
     switch (kt) {
-      case 1:
-        if (!nt) return &launch_one<P, W, SPLIT, OUT, 1, 8, false>;
-        if (u == 4) return &launch_one<P, W, SPLIT, OUT, 1, 4, true>;
-        if (u == 16) return &launch_one<P, W, SPLIT, OUT, 1, 16, true>;
-        return &launch_one<P, W, SPLIT, OUT, 1, 8, true>;
-      case 2: return &launch_one<P, W, SPLIT, OUT, 2, 4, true>;
-      case 4: return &launch_one<P, W, SPLIT, OUT, 4, 2, true>;
-      case 8: return &launch_one<P, W, SPLIT, OUT, 8, 2, true>;
-      case 16: return &launch_one<P, W, SPLIT, OUT, 16, 1, true>;
-      case 32: return &launch_one<P, W, SPLIT, OUT, 32, 1, true>;
+      case 2: return &launch_one<P, W, SPLIT, OUT, 2, 4, true, 1>;
+      case 4: return &launch_one<P, W, SPLIT, OUT, 4, 2, true, 1>;
+      case 8: return &launch_one<P, W, SPLIT, OUT, 8, 2, true, 1>;
+      case 16: return &launch_one<P, W, SPLIT, OUT, 16, 1, true, 1>;
+      case 32: return &launch_one<P, W, SPLIT, OUT, 32, 1, true, 1>;
     }
     return nullptr;
   }
 }
 
 template <class P, int W, bool SPLIT>
-LaunchFn pick_out(int out, int kt, int u, bool nt) {
-  if (out == OUT_RUN) return pick_kt<P, W, SPLIT, OUT_RUN>(kt, u, nt);
-  if (out == OUT_RUNX) return pick_kt<P, W, SPLIT, OUT_RUNX>(kt, u, nt);
-  if constexpr (std::is_same_v<P, PF64>) return pick_kt<P, W, SPLIT, OUT_RESID>(kt, u, nt);
+LaunchFn pick_out(int out, int kt, int u, bool nt, int r) {
+  if (out == OUT_RUN) return pick_kt<P, W, SPLIT, OUT_RUN>(kt, u, nt, r);
+  if (out == OUT_RUNX) return pick_kt<P, W, SPLIT, OUT_RUNX>(kt, u, nt, r);
+  if constexpr (std::is_same_v<P, PF64>) return pick_kt<P, W, SPLIT, OUT_RESID>(kt, u, nt, r);
   return nullptr;  // the residual is FP64 only
 }
 
 template <class P, int W>
-LaunchFn pick_split(bool split, int out, int kt, int u, bool nt) {
-  return split ? pick_out<P, W, true>(out, kt, u, nt) : pick_out<P, W, false>(out, kt, u, nt);
+LaunchFn pick_split(bool split, int out, int kt, int u, bool nt, int r) {
+  return split ? pick_out<P, W, true>(out, kt, u, nt, r) : pick_out<P, W, false>(out, kt, u, nt, r);
 }
 
-LaunchFn pick(int prec, int W, bool split, int out, int kt, int u, bool nt) {
+LaunchFn pick(int prec, int W, bool split, int out, int kt, int u, bool nt, int r) {
   // prec: 0 FP64, 1 FP32 compensated, 2 FP32 with FP64 accumulation
   if (prec == 0)
-    return W == 64 ? pick_split<PF64, 64>(split, out, kt, u, nt)
-                   : pick_split<PF64, 32>(split, out, kt, u, nt);
+    return W == 64 ? pick_split<PF64, 64>(split, out, kt, u, nt, r)
+                   : pick_split<PF64, 32>(split, out, kt, u, nt, r);
   if (W != 64) return nullptr;
-  if (prec == 1) return pick_split<PF32C, 64>(split, out, kt, u, nt);
-  return pick_split<PF32D, 64>(split, out, kt, u, nt);
+  if (prec == 1) return pick_split<PF32C, 64>(split, out, kt, u, nt, r);
+  return pick_split<PF32D, 64>(split, out, kt, u, nt, r);
+}
+
+using StagedFn = void (*)(const SpmvArgs &, dim3, dim3, size_t, int32_t, int32_t, hipStream_t);
+
+template <class P, bool SPLIT, int OUT, int U>
+void launch_staged(const SpmvArgs &a, dim3 grid, dim3 block, size_t lds, int32_t nwork,
+                   int32_t remap, hipStream_t s) {
+  hipLaunchKernelGGL((spmv_staged_kernel<P, SPLIT, OUT, U>), grid, block, lds, s, a, nwork, remap);
+}
+
+template <class P, bool SPLIT>
+StagedFn pick_staged_out(int out, int u) {
+  if (out == OUT_RUN) return u == 16 ? &launch_staged<P, SPLIT, OUT_RUN, 16> : &launch_staged<P, SPLIT, OUT_RUN, 8>;
+  if (out == OUT_RUNX) return u == 16 ? &launch_staged<P, SPLIT, OUT_RUNX, 16> : &launch_staged<P, SPLIT, OUT_RUNX, 8>;
+  if constexpr (std::is_same_v<P, PF64>)
+    return u == 16 ? &launch_staged<P, SPLIT, OUT_RESID, 16> : &launch_staged<P, SPLIT, OUT_RESID, 8>;
+  return nullptr;
+}
+
+template <class P>
+StagedFn pick_staged_split(bool split, int out, int u) {
+  return split ? pick_staged_out<P, true>(out, u) : pick_staged_out<P, false>(out, u);
+}
+
+StagedFn pick_staged(int prec, bool split, int out, int u) {
+  if (u != 8 && u != 16) return nullptr;
+  if (prec == 0) return pick_staged_split<PF64>(split, out, u);
+  if (prec == 1) return pick_staged_split<PF32C>(split, out, u);
+  return pick_staged_split<PF32D>(split, out, u);
+}
+
+size_t staged_lds_per_wave(int prec, int u) {
+  return (size_t)u * 64 * ((prec == 0 ? sizeof(double) : sizeof(float)) + sizeof(int32_t));
 }
 
 int floor_pow2(int v) {
@@ -396,10 +607,17 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
   a.nznrows = D.nznrows;
   a.nrows = D.nrows;
   const bool split = D.variant == 2;
-  const int gpw = D.W == 64 ? 1 : 2;
-  const int64_t waves = ((int64_t)D.ngroups + gpw - 1) / gpw;
-  const int64_t ngb = (waves + wpb - 1) / wpb;
-  const int64_t nfill = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+  const int rpl = cfg.rows_per_lane;
+  if (rpl != 1 && rpl != 2 && rpl != 4) fail(WK_ERR_ARG, "rows_per_lane must be 1, 2 or 4");
+  // dynamic LDS only caps the number of resident blocks per CU
+  size_t lds = 0;
+  if (cfg.blocks_per_cu > 0) {
+    int dev = 0, per_cu = 0, per_block = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev), "attr");
+    hip_check(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
+    lds = std::min<size_t>((size_t)per_cu / (size_t)cfg.blocks_per_cu, (size_t)per_block);
+  }
 
   // k = sum of power-of-two tiles: as many k_tile tiles as fit, then the rest
   int32_t j0 = 0;
@@ -409,8 +627,40 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const int32_t rem = k - j0;
     while (kt > rem) kt >>= 1;
     const int32_t ntiles = rem / kt;
-    LaunchFn fn = pick(prec, D.W, split, out, kt, cfg.unroll, cfg.nontemporal != 0);
+    // rows per lane applies to single right-hand-side tiles of W = 64 layouts
+    const int r = (kt == 1 && D.W == 64) ? rpl : 1;
+    if (cfg.stage_lds && kt == 1 && D.W == 64 && r == 1) {
+      StagedFn sf = pick_staged(prec, split, out, cfg.unroll);
+      if (!sf) fail(WK_ERR_ARG, "LDS staging needs unroll 8 or 16");
+      const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
+      const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+      const size_t need = staged_lds_per_wave(prec, cfg.unroll) * (size_t)wpb;
+      {
+        int dev = 0, per_block = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
+        if (need > (size_t)per_block) fail(WK_ERR_ARG, "LDS staging: waves_per_block * unroll too large");
+      }
+      for (int32_t t = 0; t < ntiles; t++) {
+        a.x = static_cast<const char *>(x) + (size_t)(j0 + t) * xsz;
+        a.y = y + j0 + t;
+        a.b = b ? b + j0 + t : nullptr;
+        a.k = 1;
+        if (nb + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+        if (nb + nf > 0)
+          sf(a, dim3((uint32_t)(nb + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nb,
+             cfg.xcd_remap ? 1 : 0, stream);
+        hip_check(hipGetLastError(), "spmv launch");
+      }
+      j0 += ntiles;
+      continue;
+    }
+    LaunchFn fn = pick(prec, D.W, split, out, kt, cfg.unroll, cfg.nontemporal != 0, r);
     if (!fn) fail(WK_ERR_ARG, "unsupported kernel configuration");
+    const int gpw = D.W == 64 ? r : 2;  // groups per wave
+    const int64_t waves = ((int64_t)D.ngroups + gpw - 1) / gpw;
+    const int64_t ngb = (waves + wpb - 1) / wpb;
+    const int64_t nfill = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
     const int64_t nwork = ngb * ntiles;
     if (nwork + nfill > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
     a.x = static_cast<const char *>(x) + (size_t)j0 * xsz;
@@ -418,7 +668,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     a.b = b ? b + j0 : nullptr;
     a.k = ntiles * kt;
     if (nwork + nfill > 0)
-      fn(a, dim3((uint32_t)(nwork + nfill)), dim3(wpb * 64), (int32_t)nwork, ntiles,
+      fn(a, dim3((uint32_t)(nwork + nfill)), dim3(wpb * 64), lds, (int32_t)nwork, ntiles,
          cfg.xcd_remap ? 1 : 0, stream);
     hip_check(hipGetLastError(), "spmv launch");
     j0 += ntiles * kt;

@@ -1,0 +1,192 @@
+// tune.cpp -- per-engine launch configurations and the autotuner.
+//
+// The reference tunes by sweeping warps/block (1..16) and the WPK2 threshold
+// in its drivers and keeping the fastest (wpk1.cu:138-257, wpk2.cu:156-302).
+// Here every engine keeps one launch configuration per (operation, k): a
+// layout-based choice until wk_tune measures the candidates on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "wk_internal.hpp"
+
+namespace wk {
+
+namespace {
+
+int64_t cfg_key(int32_t op, int32_t k) { return ((int64_t)op << 32) | (uint32_t)k; }
+
+int pow2_at_most(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+
+const DevLayout &layout_for(const wk_engine *E, int32_t op) { return op == WK_OP_ADJOINT ? E->adj : E->fwd; }
+
+// Candidate set: the knobs that change speed but not results.
+std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_launch &base, int32_t k) {
+  std::vector<wk_launch> out;
+  const size_t vsz = dtype == WK_F64 ? 8 : 4;
+  for (int wpb : {1, 2, 4, 8})
+    for (int remap : {0, 1}) {
+      wk_launch c = base;
+      c.waves_per_block = wpb;
+      c.xcd_remap = remap;
+      c.blocks_per_cu = 0;
+      if (k == 1 || D.W == 32) {
+        c.k_tile = 1;
+        for (int u : {8, 16}) {
+          c.unroll = u;
+          c.rows_per_lane = 1;
+          c.stage_lds = 0;
+          out.push_back(c);
+          if (D.W == 64 && (size_t)wpb * u * 64 * (vsz + 4) <= 65536) {
+            c.stage_lds = 1;
+            out.push_back(c);
+            c.stage_lds = 0;
+          }
+        }
+        if (D.W == 64) {
+          c.unroll = 8;
+          c.rows_per_lane = 2;
+          out.push_back(c);
+        }
+      } else {
+        c.unroll = 8;
+        c.rows_per_lane = 1;
+        c.stage_lds = 0;
+        for (int kt = 1; kt <= std::min(32, pow2_at_most(k)); kt *= 2) {
+          c.k_tile = kt;
+          out.push_back(c);
+        }
+      }
+    }
+  return out;
+}
+
+}  // namespace
+
+wk_launch heuristic_launch(const DevLayout &D, int32_t k) {
+  wk_launch c;
+  wk_launch_default(&c);
+  const double mean_len =
+      D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * D.W) : 0.0;
+  if (k == 1 && D.W == 64 && mean_len >= 40.0) {
+    // long rows: 16-byte batch copies through LDS (measured on hex elasticity)
+    c.stage_lds = 1;
+    c.unroll = 16;
+    c.waves_per_block = 4;
+    c.xcd_remap = 0;
+  } else {
+    c.stage_lds = 0;
+    c.unroll = 8;
+    c.waves_per_block = 2;
+    c.xcd_remap = 1;
+    c.k_tile = std::min(8, pow2_at_most(std::max(1, k)));
+  }
+  return c;
+}
+
+wk_launch engine_launch(wk_engine *E, int32_t op, int32_t k) {
+  {
+    std::lock_guard<std::mutex> g(E->cfg_mu);
+    auto it = E->launch_cfg.find(cfg_key(op, k));
+    if (it != E->launch_cfg.end()) return it->second;
+  }
+  return heuristic_launch(layout_for(E, op), k);
+}
+
+}  // namespace wk
+
+using namespace wk;
+
+extern "C" {
+
+int wk_engine_get_launch(const wk_engine *E, int32_t op, int32_t k, wk_launch *cfg) {
+  if (!E || !cfg || op < 0 || op > 2 || k < 1) {
+    set_last_error("bad argument");
+    return WK_ERR_ARG;
+  }
+  *cfg = engine_launch(const_cast<wk_engine *>(E), op, k);
+  return WK_OK;
+}
+
+int wk_engine_set_launch(wk_engine *E, int32_t op, int32_t k, const wk_launch *cfg) {
+  if (!E || !cfg || op < 0 || op > 2 || k < 1) {
+    set_last_error("bad argument");
+    return WK_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> g(E->cfg_mu);
+  E->launch_cfg[cfg_key(op, k)] = *cfg;
+  return WK_OK;
+}
+
+int wk_tune(wk_engine *E, int32_t op, const wk_launch *base, const void *x_dev, double *y_dev,
+            int32_t k, int64_t ldx, int64_t ldy, int32_t reps, void *stream, wk_launch *best,
+            float *best_ms) {
+  if (!E || op < 0 || op > 2 || k < 1 || reps < 1) {
+    set_last_error("bad argument");
+    return WK_ERR_ARG;
+  }
+  if (op == WK_OP_ADJOINT && !E->has_adj) {
+    set_last_error("engine was created without WK_ENGINE_ADJOINT");
+    return WK_ERR_STATE;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  wk_launch b0 = base ? *base : engine_launch(E, op, k);
+  const DevLayout &D = layout_for(E, op);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    set_last_error("hipEventCreate failed");
+    return WK_ERR_HIP;
+  }
+  float best_t = INFINITY;
+  wk_launch best_c = b0;
+  int rc = WK_OK;
+  for (const wk_launch &c : candidates(D, E->dtype, b0, k)) {
+    auto launch = [&]() {
+      if (op == WK_OP_RUN) return wk_run(E, &c, x_dev, y_dev, k, ldx, ldy, stream);
+      if (op == WK_OP_RUN_X) return wk_run_x(E, &c, x_dev, y_dev, k, ldx, ldy, stream);
+      return wk_run_adjoint(E, &c, x_dev, y_dev, k, ldx, ldy, stream);
+    };
+    if ((rc = launch()) != WK_OK) {
+      if (rc == WK_ERR_ARG) continue;  // shape not offered for this layout
+      break;
+    }
+    for (int w = 0; w < 2 && rc == WK_OK; w++) rc = launch();
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == WK_OK; r++) rc = launch();
+    (void)hipEventRecord(e1, s);
+    if (rc != WK_OK) break;
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = WK_ERR_HIP;
+      set_last_error("tuning timing failed");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)reps;
+    if (ms < best_t) {
+      best_t = ms;
+      best_c = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != WK_OK && rc != WK_ERR_ARG) return rc;
+  if (!std::isfinite(best_t)) {
+    set_last_error("no launch configuration ran");
+    return WK_ERR_ARG;
+  }
+  {
+    std::lock_guard<std::mutex> g(E->cfg_mu);
+    E->launch_cfg[cfg_key(op, k)] = best_c;
+  }
+  if (best) *best = best_c;
+  if (best_ms) *best_ms = best_t;
+  return WK_OK;
+}
+
+}  // extern "C"

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -90,6 +92,8 @@ struct wk_engine {
   int32_t *d_colind = nullptr;
   std::vector<void *> owned;        // every hipMalloc of this engine
   int device = 0;
+  std::mutex cfg_mu;                // guards launch_cfg
+  std::map<int64_t, wk_launch> launch_cfg;  // (op << 32 | k) -> configuration
 };
 
 namespace wk {
@@ -125,5 +129,10 @@ void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
 void launch_dvals(const int32_t *rowptr, const int32_t *colind, int32_t nrows, int64_t nnz,
                   const double *dy, int64_t lddy, const double *x, int64_t ldx, int32_t k,
                   double *dvals, hipStream_t stream);
+
+// The configuration an engine uses for (op, k) when the caller passes none:
+// the tuned one if any, else a choice from the layout.
+wk_launch engine_launch(wk_engine *E, int32_t op, int32_t k);
+wk_launch heuristic_launch(const DevLayout &D, int32_t k);
 
 }  // namespace wk
