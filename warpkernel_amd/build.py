@@ -66,6 +66,19 @@ def build_fegen() -> str:
     return out
 
 
+def build_dropin_test() -> str:
+    """tests/cpp/dropin_wpk: the reference drivers' flow through include/warpkernel.hpp,
+    checked against the oracle (links the product, the generators and the oracle)."""
+    out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "dropin_wpk")
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_wpk.cpp")
+    _run([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-o", out, src,
+          f"-L{PKG}", "-lwarpkernel", "-lwk_fegen", f"-L{os.path.join(ROOT, 'oracle')}", "-lwk_oracle",
+          "-Wl,-rpath,$ORIGIN/../../../warpkernel_amd", "-Wl,-rpath,$ORIGIN/../../../oracle"])
+    return out
+
+
 def build_all() -> None:
     build_lib()
     build_fegen()
