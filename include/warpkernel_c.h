@@ -221,6 +221,42 @@ int wk_tune(wk_engine *engine, int32_t op, const wk_launch *base, const void *x_
 int wk_engine_get_launch(const wk_engine *engine, int32_t op, int32_t k, wk_launch *cfg);
 int wk_engine_set_launch(wk_engine *engine, int32_t op, int32_t k, const wk_launch *cfg);
 
+/* ------------------------------------------------------ batches and graphs */
+
+/* Many matrices in one persistent launch (no reference counterpart; the
+ * reference launches one kernel per matrix and per call).  The groups of all
+ * engines (and the chunks of their empty rows) form one work list, split into
+ * contiguous ranges of equal cost over a fixed grid of persistent waves;
+ * neighbouring groups stay on one wave, no atomics.  Engines must share the
+ * value type, use group width 64 and outlive the batch. */
+typedef struct wk_batch wk_batch;
+
+/* waves_per_cu: persistent waves per compute unit (0: 8). */
+int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
+                    wk_batch **out);
+
+/* Binds the vectors of every matrix: x_devs[i] ([ncols_i][ldx[i]]) and y_devs[i]
+ * ([nrows_i][ldy[i]]) for k right-hand sides (1..32).  Host arrays of device
+ * pointers; copied to the device here (synchronously), so bind outside graph
+ * capture and re-bind when the vectors move. */
+int wk_batch_bind(wk_batch *batch, const void *const *x_devs, double *const *y_devs, int32_t k,
+                  const int64_t *ldx, const int64_t *ldy);
+
+/* y_i = A_i x_i for every matrix of the batch, one kernel launch on `stream`. */
+int wk_batch_run(wk_batch *batch, void *stream);
+
+void wk_batch_destroy(wk_batch *batch);
+
+/* HIP graph of any sequence of launches of this library (wk_run, wk_run_x,
+ * wk_residual, wk_run_adjoint, wk_dvals, wk_engine_update_values,
+ * wk_batch_run) on one stream: begin capture, issue the calls, end.  None of
+ * those calls allocates, copies from the host or sets memory. */
+typedef struct wk_graph wk_graph;
+int wk_graph_begin(void *stream);
+int wk_graph_end(void *stream, wk_graph **out);
+int wk_graph_launch(wk_graph *graph, void *stream);
+void wk_graph_destroy(wk_graph *graph);
+
 /* Blocking timing helper with the reference's contract (engine::run returns
  * the seconds of one launch, warpkernel.hpp:829-843): launches once, waits,
  * returns the elapsed seconds measured with HIP events in *seconds.

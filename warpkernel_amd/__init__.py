@@ -8,6 +8,6 @@ binds it with ctypes:
   warpkernel_amd.autograd                          torch.autograd.Function
 """
 from ._capi import WKError, default_launch, lib  # noqa: F401
-from .api import engine, structure, structure2  # noqa: F401
+from .api import batch, engine, graph, structure, structure2  # noqa: F401
 
-__all__ = ["structure", "structure2", "engine", "WKError", "default_launch", "lib"]
+__all__ = ["structure", "structure2", "engine", "batch", "graph", "WKError", "default_launch", "lib"]

@@ -352,3 +352,88 @@ def _verify(orig, comp, row_map) -> bool:
         print(f"verify: {o[i]}\t{c[i]}\t{i}")
         return False
     return True
+
+
+class batch:
+    """Many engines in one persistent launch (wk_batch_*).
+
+        b = batch([eng0, eng1, ...]);  b.bind(xs, ys);  b.run()
+
+    xs / ys: lists of device tensors, [ncols_i] / [nrows_i] or [n, k]."""
+
+    def __init__(self, engines, waves_per_cu: int = 0):
+        self.engines = list(engines)  # keep them alive
+        arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        h = C.c_void_p()
+        check(lib().wk_batch_create(arr, len(self.engines), int(waves_per_cu), C.byref(h)))
+        self._h = h
+
+    def bind(self, xs, ys):
+        n = len(self.engines)
+        if len(xs) != n or len(ys) != n:
+            raise ValueError("one x and one y per engine")
+        k, _ = engine._k_ld(xs[0])
+        xp = (C.c_void_p * n)(*[_dev_ptr(t) for t in xs])
+        yp = (C.c_void_p * n)(*[_dev_ptr(t) for t in ys])
+        ldx = (C.c_int64 * n)(*[engine._k_ld(t)[1] for t in xs])
+        ldy = (C.c_int64 * n)(*[engine._k_ld(t)[1] for t in ys])
+        check(lib().wk_batch_bind(self._h, xp, yp, k, ldx, ldy))
+        self._bound = (list(xs), list(ys))
+
+    def run(self, stream=None):
+        check(lib().wk_batch_run(self._h, _stream(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().wk_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class graph:
+    """HIP graph of library launches captured on a (non-null) stream:
+
+        with graph.capture(stream) as g:
+            eng.launch(x, y, stream=stream); b.run(stream)
+        g.launch(stream)
+    """
+
+    def __init__(self):
+        self._h = None
+
+    @classmethod
+    def capture(cls, stream):
+        g = cls()
+        g._stream = stream
+        return g
+
+    def __enter__(self):
+        check(lib().wk_graph_begin(_stream(self._stream)))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        h = C.c_void_p()
+        rc = lib().wk_graph_end(_stream(self._stream), C.byref(h))
+        if et is None:
+            check(rc)
+            self._h = h
+        return False
+
+    def launch(self, stream=None):
+        check(lib().wk_graph_launch(self._h, _stream(stream if stream is not None else self._stream)))
+
+    def close(self):
+        if self._h is not None:
+            lib().wk_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

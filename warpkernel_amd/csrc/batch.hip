@@ -1,0 +1,414 @@
+// batch.hip -- many WPK matrices in one persistent launch, and HIP graph
+// capture of any sequence of this library's launches.
+//
+// Work list.  Every group of every matrix is one item (cost: its steps times
+// the k-tiles), every 64 rows without nonzeros one fill item.  Items keep the
+// matrices' order and, within a matrix, the layout's group order; the list is
+// cut into nwaves contiguous ranges of equal cost, so a wave streams
+// neighbouring groups (x locality) and waves finish together.  The grid is a
+// fixed number of persistent waves (waves_per_cu per compute unit); there are
+// no atomics and no inter-wave communication.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "wk_internal.hpp"
+
+namespace wk {
+
+struct BatchDesc {
+  const void *vals;
+  const int32_t *cols;
+  const int32_t *row_map;
+  const int64_t *warp_offsets;
+  const uint32_t *max_nz_rows;
+  const uint32_t *tpr;
+  const uint32_t *row_off;
+  const void *x;
+  double *y;
+  int64_t ldx, ldy;
+  int32_t nznrows, nrows, ngroups, split;
+};
+
+struct BatchItem {
+  int32_t m;  // matrix
+  int32_t g;  // group (>= 0) or fill chunk -1-g
+};
+
+}  // namespace wk
+
+struct wk_batch {
+  std::vector<wk_engine *> engines;
+  int32_t dtype = WK_F64;
+  int device = 0;
+  int32_t nwaves = 0;
+  int32_t k = 0;
+  int32_t kt = 1;
+  std::vector<wk::BatchDesc> desc;
+  wk::BatchDesc *d_desc = nullptr;
+  wk::BatchItem *d_items = nullptr;
+  int32_t *d_wstart = nullptr;
+  int64_t nitems = 0;
+};
+
+struct wk_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace wk {
+namespace {
+
+// The per-precision arithmetic of kernels.hip, repeated here for this
+// translation unit (device functions are not shared across objects without
+// relocatable device code).
+struct BF64 {
+  using V = double;
+  using X = double;
+  struct Acc {
+    double s;
+  };
+  __device__ static void first(Acc &a, V v, X x) { a.s = v * x; }
+  __device__ static void step(Acc &a, V v, X x) { a.s = __builtin_fma(v, x, a.s); }
+  __device__ static void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
+  __device__ static Acc shfl_down(const Acc &a, int o) { return Acc{__shfl_down(a.s, o, 64)}; }
+  __device__ static double result(const Acc &a) { return a.s; }
+};
+
+struct BF32C {
+  using V = float;
+  using X = float;
+  struct Acc {
+    float s, c;
+  };
+  __device__ static void two_sum(float a, float b, float &s, float &t) {
+    s = a + b;
+    float bp = s - a;
+    float ap = s - bp;
+    t = (a - ap) + (b - bp);
+  }
+  __device__ static void first(Acc &a, V v, X x) {
+    a.s = v * x;
+    a.c = __builtin_fmaf(v, x, -a.s);
+  }
+  __device__ static void step(Acc &a, V v, X x) {
+    float p = v * x, e = __builtin_fmaf(v, x, -p), s, t;
+    two_sum(a.s, p, s, t);
+    a.s = s;
+    a.c = a.c + (t + e);
+  }
+  __device__ static void add(Acc &a, const Acc &b) {
+    float s, t;
+    two_sum(a.s, b.s, s, t);
+    a.s = s;
+    a.c = (a.c + b.c) + t;
+  }
+  __device__ static Acc shfl_down(const Acc &a, int o) {
+    return Acc{__shfl_down(a.s, o, 64), __shfl_down(a.c, o, 64)};
+  }
+  __device__ static double result(const Acc &a) { return (double)a.s + (double)a.c; }
+};
+
+struct BF32D {
+  using V = float;
+  using X = float;
+  struct Acc {
+    double s;
+  };
+  __device__ static void first(Acc &a, V v, X x) { a.s = (double)v * (double)x; }
+  __device__ static void step(Acc &a, V v, X x) { a.s = __builtin_fma((double)v, (double)x, a.s); }
+  __device__ static void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
+  __device__ static Acc shfl_down(const Acc &a, int o) { return Acc{__shfl_down(a.s, o, 64)}; }
+  __device__ static double result(const Acc &a) { return a.s; }
+};
+
+// One group of one matrix for right-hand sides [j0, j0+KT): the summation
+// order of warpKernel / warpKernel2 (first product, then fma per slot; the
+// pairwise lane tree for split rows).
+template <class P, int KT, int U>
+__device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32_t j0, int32_t lane) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  const int64_t base = d.warp_offsets[g];
+  const uint32_t len = d.max_nz_rows[g];
+  const uint32_t tpr = d.split ? d.tpr[g] : 1u;
+  const uint32_t row0 = d.split ? d.row_off[g] : (uint32_t)g * 64u;
+  const V *vp = static_cast<const V *>(d.vals) + base + lane;
+  const int32_t *cp = d.cols + base + lane;
+  const X *xb = static_cast<const X *>(d.x) + j0;
+  Acc acc[KT];
+  {
+    const int32_t c = cp[0];
+    const V v = __builtin_nontemporal_load(vp);
+#pragma unroll
+    for (int j = 0; j < KT; j++) P::first(acc[j], v, xb[(int64_t)c * d.ldx + j]);
+  }
+  uint32_t i = 1;
+  for (; i + U <= len; i += U) {
+    int32_t c[U];
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      c[u] = cp[(int64_t)(i + u) * 64];
+      v[u] = __builtin_nontemporal_load(vp + (int64_t)(i + u) * 64);
+    }
+    X xv[U][KT];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int j = 0; j < KT; j++) xv[u][j] = xb[(int64_t)c[u] * d.ldx + j];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
+  }
+  for (; i < len; i++) {
+    const int32_t c = cp[(int64_t)i * 64];
+    const V v = __builtin_nontemporal_load(vp + (int64_t)i * 64);
+#pragma unroll
+    for (int j = 0; j < KT; j++) P::step(acc[j], v, xb[(int64_t)c * d.ldx + j]);
+  }
+  for (uint32_t o = 1; o < tpr; o <<= 1) {
+#pragma unroll
+    for (int j = 0; j < KT; j++) {
+      Acc other = P::shfl_down(acc[j], (int)o);
+      P::add(acc[j], other);
+    }
+  }
+  const uint32_t rowid = row0 + (uint32_t)lane / tpr;
+  if (((uint32_t)lane & (tpr - 1)) != 0 || rowid >= (uint32_t)d.nznrows) return;
+  double *yr = d.y + (int64_t)d.row_map[rowid] * d.ldy + j0;
+#pragma unroll
+  for (int j = 0; j < KT; j++) yr[j] = P::result(acc[j]);
+}
+
+template <class P, int KT, int U>
+__global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict__ desc,
+                                                    const BatchItem *__restrict__ items,
+                                                    const int32_t *__restrict__ wstart,
+                                                    int32_t nwaves, int32_t ktiles) {
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t w = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (w >= nwaves) return;
+  const int32_t it1 = wstart[w + 1];
+  for (int32_t it = wstart[w]; it < it1; it++) {
+    const BatchItem item = items[it];
+    const BatchDesc d = desc[item.m];
+    if (item.g < 0) {
+      const int64_t pos = (int64_t)d.nznrows + (int64_t)(-1 - item.g) * 64 + lane;
+      if (pos < d.nrows) {
+        double *yr = d.y + (int64_t)d.row_map[pos] * d.ldy;
+        for (int32_t j = 0; j < ktiles * KT; j++) yr[j] = 0.0;
+      }
+      continue;
+    }
+    for (int32_t kt = 0; kt < ktiles; kt++) batch_group<P, KT, U>(d, item.g, kt * KT, lane);
+  }
+}
+
+using BatchFn = void (*)(const wk_batch *, int32_t, hipStream_t);
+
+template <class P, int KT>
+void launch_batch(const wk_batch *B, int32_t ktiles, hipStream_t s) {
+  const int32_t blocks = (B->nwaves + 3) / 4;
+  hipLaunchKernelGGL((batch_kernel<P, KT, 8>), dim3(blocks), dim3(256), 0, s, B->d_desc, B->d_items,
+                     B->d_wstart, B->nwaves, ktiles);
+}
+
+template <class P>
+BatchFn pick_batch_kt(int kt) {
+  switch (kt) {
+    case 1: return &launch_batch<P, 1>;
+    case 2: return &launch_batch<P, 2>;
+    case 4: return &launch_batch<P, 4>;
+    case 8: return &launch_batch<P, 8>;
+  }
+  return nullptr;
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return WK_OK;
+  } catch (const Error &e) {
+    set_last_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return WK_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return WK_ERR_ARG;
+  }
+}
+
+void free_batch(wk_batch *B) {
+  if (!B) return;
+  (void)hipFree(B->d_desc);
+  (void)hipFree(B->d_items);
+  (void)hipFree(B->d_wstart);
+  delete B;
+}
+
+}  // namespace
+}  // namespace wk
+
+using namespace wk;
+
+extern "C" {
+
+int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu, wk_batch **out) {
+  return guarded([&] {
+    if (!out || !engines || count < 1) fail(WK_ERR_ARG, "bad argument");
+    *out = nullptr;
+    if (waves_per_cu < 0 || waves_per_cu > 32) fail(WK_ERR_ARG, "waves_per_cu must be in [0, 32]");
+    auto *B = new wk_batch();
+    try {
+      hip_check(hipGetDevice(&B->device), "hipGetDevice");
+      for (int32_t i = 0; i < count; i++) {
+        wk_engine *E = engines[i];
+        if (!E) fail(WK_ERR_ARG, "NULL engine in batch");
+        if (E->fwd.W != 64) fail(WK_ERR_ARG, "batched engines need group width 64");
+        if (i && E->dtype != engines[0]->dtype) fail(WK_ERR_ARG, "batched engines must share the value type");
+        if (E->device != B->device) fail(WK_ERR_ARG, "engine lives on another device");
+        B->engines.push_back(E);
+      }
+      B->dtype = engines[0]->dtype;
+      int cus = 0;
+      hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device), "attr");
+      B->nwaves = cus * (waves_per_cu ? waves_per_cu : 8);
+      B->desc.resize(count);
+      hip_check(hipMalloc(&B->d_desc, sizeof(BatchDesc) * count), "hipMalloc");
+      hip_check(hipMalloc(&B->d_wstart, sizeof(int32_t) * (B->nwaves + 1)), "hipMalloc");
+    } catch (...) {
+      free_batch(B);
+      throw;
+    }
+    *out = B;
+  });
+}
+
+int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs, int32_t k,
+                  const int64_t *ldx, const int64_t *ldy) {
+  return guarded([&] {
+    if (!B || !x_devs || !y_devs || !ldx || !ldy) fail(WK_ERR_ARG, "NULL argument");
+    if (k < 1 || k > 32) fail(WK_ERR_ARG, "k must be in [1, 32]");
+    const int32_t kt = k % 8 == 0 ? 8 : k % 4 == 0 ? 4 : k % 2 == 0 ? 2 : 1;
+    const int32_t ktiles = k / kt;
+    std::vector<BatchItem> items;
+    std::vector<int64_t> cost;
+    for (size_t i = 0; i < B->engines.size(); i++) {
+      const wk_engine *E = B->engines[i];
+      const DevLayout &D = E->fwd;
+      if (ldx[i] < k || ldy[i] < k) fail(WK_ERR_ARG, "leading dimension smaller than k");
+      BatchDesc &d = B->desc[i];
+      d.vals = D.vals;
+      d.cols = D.cols;
+      d.row_map = D.row_map;
+      d.warp_offsets = D.warp_offsets;
+      d.max_nz_rows = D.max_nz_rows;
+      d.tpr = D.tpr;
+      d.row_off = D.row_off;
+      d.x = x_devs[i];
+      d.y = y_devs[i];
+      d.ldx = ldx[i];
+      d.ldy = ldy[i];
+      d.nznrows = D.nznrows;
+      d.nrows = D.nrows;
+      d.ngroups = D.ngroups;
+      d.split = D.variant == 2;
+      if ((D.nrows && !y_devs[i]) || (E->ncols && !x_devs[i])) fail(WK_ERR_ARG, "NULL vector");
+      for (int32_t g = 0; g < D.ngroups; g++) {
+        items.push_back({(int32_t)i, g});
+        cost.push_back((int64_t)D.h_max_nz_rows[g] * ktiles);
+      }
+      const int32_t nfill = (D.nrows - D.nznrows + 63) / 64;
+      for (int32_t c = 0; c < nfill; c++) {
+        items.push_back({(int32_t)i, -1 - c});
+        cost.push_back(1);
+      }
+    }
+    // contiguous ranges of equal cost
+    int64_t total = 0;
+    for (int64_t c : cost) total += c;
+    std::vector<int32_t> wstart(B->nwaves + 1, 0);
+    int64_t acc = 0;
+    int32_t w = 0;
+    for (size_t it = 0; it < items.size(); it++) {
+      while (w < B->nwaves && acc >= total * (w + 1) / B->nwaves) wstart[++w] = (int32_t)it;
+      acc += cost[it];
+    }
+    while (w < B->nwaves) wstart[++w] = (int32_t)items.size();
+    (void)hipFree(B->d_items);
+    B->d_items = nullptr;
+    if (!items.empty())
+      hip_check(hipMalloc(&B->d_items, sizeof(BatchItem) * items.size()), "hipMalloc");
+    if (!items.empty())
+      hip_check(hipMemcpy(B->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice), "bind");
+    hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(), hipMemcpyHostToDevice), "bind");
+    hip_check(hipMemcpy(B->d_wstart, wstart.data(), sizeof(int32_t) * wstart.size(), hipMemcpyHostToDevice), "bind");
+    B->nitems = (int64_t)items.size();
+    B->k = k;
+    B->kt = kt;
+  });
+}
+
+int wk_batch_run(wk_batch *B, void *stream) {
+  return guarded([&] {
+    if (!B) fail(WK_ERR_ARG, "batch is NULL");
+    if (!B->k) fail(WK_ERR_STATE, "batch has no bound vectors (wk_batch_bind)");
+    if (!B->nitems) return;
+    BatchFn fn = B->dtype == WK_F64 ? pick_batch_kt<BF64>(B->kt) : pick_batch_kt<BF32C>(B->kt);
+    if (!fn) fail(WK_ERR_ARG, "unsupported batch configuration");
+    fn(B, B->k / B->kt, static_cast<hipStream_t>(stream));
+    hip_check(hipGetLastError(), "batch launch");
+  });
+}
+
+void wk_batch_destroy(wk_batch *B) { free_batch(B); }
+
+int wk_graph_begin(void *stream) {
+  return guarded([&] {
+    if (!stream) fail(WK_ERR_ARG, "graph capture needs a non-null stream");
+    hip_check(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal),
+              "hipStreamBeginCapture");
+  });
+}
+
+int wk_graph_end(void *stream, wk_graph **out) {
+  return guarded([&] {
+    if (!stream || !out) fail(WK_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    auto *G = new wk_graph();
+    hipError_t e = hipStreamEndCapture(static_cast<hipStream_t>(stream), &G->graph);
+    if (e != hipSuccess) {
+      delete G;
+      hip_check(e, "hipStreamEndCapture");
+    }
+    e = hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(G->graph);
+      delete G;
+      hip_check(e, "hipGraphInstantiate");
+    }
+    *out = G;
+  });
+}
+
+int wk_graph_launch(wk_graph *G, void *stream) {
+  return guarded([&] {
+    if (!G) fail(WK_ERR_ARG, "graph is NULL");
+    hip_check(hipGraphLaunch(G->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+  });
+}
+
+void wk_graph_destroy(wk_graph *G) {
+  if (!G) return;
+  (void)hipGraphExecDestroy(G->exec);
+  (void)hipGraphDestroy(G->graph);
+  delete G;
+}
+
+}  // extern "C"

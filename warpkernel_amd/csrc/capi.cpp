@@ -103,6 +103,7 @@ void upload_layout(wk_engine *E, DevLayout &D, const wk_layout &L, const double 
   D.row_map = dev_upload(E, L.row_map.data(), L.nrows, s);
   D.warp_offsets = dev_upload(E, L.warp_offsets.data(), (int64_t)L.warp_offsets.size(), s);
   D.max_nz_rows = dev_upload(E, L.max_nz_rows.data(), (int64_t)L.max_nz_rows.size(), s);
+  D.h_max_nz_rows = L.max_nz_rows;
   if (L.variant == 2) {
     D.tpr = dev_upload(E, L.threads_per_row.data(), (int64_t)L.threads_per_row.size(), s);
     D.row_off = dev_upload(E, L.row_offset_warp.data(), (int64_t)L.row_offset_warp.size(), s);

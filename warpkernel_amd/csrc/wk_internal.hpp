@@ -76,6 +76,7 @@ struct DevLayout {
   uint32_t *tpr = nullptr;           // [ngroups] (variant 2)
   uint32_t *row_off = nullptr;       // [ngroups+1] (variant 2)
   uint32_t *slot_of_nz = nullptr;    // [nnz] CSR index -> padded slot (GPU process)
+  std::vector<uint32_t> h_max_nz_rows;  // host copy (batch work partition)
 };
 
 }  // namespace wk
