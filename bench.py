@@ -176,6 +176,38 @@ def extras(torch, dev, steps, warmup):
                        "fwd_adjoint_ms": round(f + a + d, 4),
                        "fwd_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (f * 1e-3) / 1e9, 1)}
     out["fwd_adjoint_hex_laplace_3d_48"] = fa
+    del eng
+
+    # 64 mixed FE matrices (2D Q1 / 3D hex elasticity, 2k-250k rows): one
+    # persistent launch vs one launch per matrix vs a replayed HIP graph
+    from warpkernel_amd import batch, graph
+    ms = fe.batch_mix(64)
+    engs = [engine(structure().scan(mm.nnz, mm.nrows, mm.vals, mm.rowptr, mm.colind), mm.vals, mm.colind)
+            for mm in ms]
+    xs = [dev(fe.uniform_x(mm.ncols, seed=7000 + i)) for i, mm in enumerate(ms)]
+    ys = [torch.empty(mm.nrows, dtype=torch.float64, device="cuda") for mm in ms]
+    tot = sum(spmv_bytes(mm.nnz, mm.nrows, mm.ncols) for mm in ms)
+    b = batch(engs)
+    b.bind(xs, ys)
+    _, mb = time_launches(torch, b.run, steps, warmup)
+
+    def each():
+        for e_, x_, y_ in zip(engs, xs, ys):
+            e_.launch(x_, y_)
+    _, me = time_launches(torch, each, steps, warmup)
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with graph.capture(st) as g:
+        b.run(stream=st)
+    with torch.cuda.stream(st):
+        _, mg = time_launches(torch, lambda: g.launch(st), steps, warmup)
+    out["batch_64_fe_matrices"] = {
+        "matrices": len(ms), "rows": int(sum(mm.nrows for mm in ms)), "nnz": int(sum(mm.nnz for mm in ms)),
+        "persistent_ms": round(mb, 4), "persistent_GBs": round(tot / (mb * 1e-3) / 1e9, 1),
+        "graph_replay_ms": round(mg, 4), "graph_replay_GBs": round(tot / (mg * 1e-3) / 1e9, 1),
+        "one_launch_per_matrix_ms": round(me, 4), "one_launch_per_matrix_GBs": round(tot / (me * 1e-3) / 1e9, 1),
+    }
+    g.close()
     return out
 
 

@@ -272,6 +272,16 @@ int wk_engine_create(const wk_layout *L, const int32_t *rowptr_host, const int32
         E->has_adj = true;
         E->d_rowptr = dev_upload(E, rowptr_host, (int64_t)L->nrows + 1, s);
         E->d_colind = dev_upload(E, colind_host, L->nz, s);
+        // dvals tiles: row of each tile's first entry (rows with entries only)
+        const int64_t nt = dvals_tiles(L->nz), ts = dvals_tile_size();
+        std::vector<int32_t> tile_row((size_t)nt + 1);
+        int32_t r = 0;
+        for (int64_t t = 0; t < nt; t++) {
+          while (r + 1 < L->nrows && (int64_t)rowptr_host[r + 1] <= t * ts) r++;
+          tile_row[t] = r;
+        }
+        tile_row[nt] = L->nrows > 0 ? L->nrows - 1 : 0;
+        E->d_tile_row = dev_upload(E, tile_row.data(), nt + 1, s);
         hip_check(hipStreamSynchronize(s), "upload");
       }
     } catch (...) {
@@ -394,8 +404,8 @@ int wk_dvals(wk_engine *E, const double *dy_dev, int64_t lddy, const double *x_d
     if (!E->has_adj) fail(WK_ERR_STATE, "engine was created without WK_ENGINE_ADJOINT");
     if (k < 1 || lddy < k || ldx < k) fail(WK_ERR_ARG, "bad k or leading dimension");
     if (E->nnz && (!dy_dev || !x_dev || !dvals_dev)) fail(WK_ERR_ARG, "NULL vector");
-    launch_dvals(E->d_rowptr, E->d_colind, E->nrows, E->nnz, dy_dev, lddy, x_dev, ldx, k,
-                 dvals_dev, static_cast<hipStream_t>(stream));
+    launch_dvals(E->d_rowptr, E->d_colind, E->d_tile_row, E->nrows, E->nnz, dy_dev, lddy, x_dev,
+                 ldx, k, dvals_dev, static_cast<hipStream_t>(stream));
   });
 }
 

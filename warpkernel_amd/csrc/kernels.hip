@@ -415,8 +415,10 @@ __global__ void scatter_kernel(const T *__restrict__ src, const uint32_t *__rest
 }
 
 // dvals[t] = sum_j dy[row_t][j] * x[col_t][j], CSR order.  Each block takes a
-// tile of entries, finds the rows the tile spans, keeps that rowptr slice in
-// LDS and binary-searches it per entry.
+// tile of kDvTile consecutive entries; tile_row[b] (built with the engine) is
+// the row holding the tile's first entry, so the block loads the rowptr slice
+// it spans into LDS and each entry finds its row by a short binary search
+// there.  Column reads and dvals writes are coalesced.
 constexpr int kDvTile = 2048;
 constexpr int kDvRows = 2048;
 
@@ -434,33 +436,24 @@ __device__ __forceinline__ int32_t upper_row(const int32_t *rp, int32_t lo, int3
 
 __global__ __launch_bounds__(256) void dvals_kernel(const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ colind,
+                                                    const int32_t *__restrict__ tile_row,
                                                     int32_t nrows, int64_t nnz,
                                                     const double *__restrict__ dy, int64_t lddy,
                                                     const double *__restrict__ x, int64_t ldx,
                                                     int32_t k, double *__restrict__ dv) {
   __shared__ int32_t srp[kDvRows + 1];
-  __shared__ int32_t sr[2];
   const int64_t e0 = (int64_t)blockIdx.x * kDvTile;
   const int64_t e1 = e0 + kDvTile < nnz ? e0 + kDvTile : nnz;
-  if (threadIdx.x == 0) {
-    // rows holding entries e0 and e1-1; skip empty rows by taking the last
-    // row whose start is <= e
-    sr[0] = upper_row(rowptr, 0, nrows - 1, e0);
-    sr[1] = upper_row(rowptr, 0, nrows - 1, e1 - 1);
-  }
-  __syncthreads();
-  const int32_t r0 = sr[0], r1 = sr[1];
+  // rows of the tile lie in [r0, r1]: the next tile starts in row r1 or later
+  const int32_t r0 = tile_row[blockIdx.x];
+  const int32_t r1 = tile_row[blockIdx.x + 1];
   const int32_t span = r1 - r0 + 1;
-  const bool in_lds = span + 1 <= kDvRows + 1;
+  const bool in_lds = span <= kDvRows;
   if (in_lds)
-    for (int32_t t = threadIdx.x; t <= span; t += blockDim.x) srp[t] = rowptr[r0 + t];
+    for (int32_t t = threadIdx.x; t < span; t += blockDim.x) srp[t] = rowptr[r0 + t];
   __syncthreads();
   for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    int32_t r;
-    if (in_lds)
-      r = r0 + upper_row(srp, 0, span - 1, e);
-    else
-      r = upper_row(rowptr, r0, r1, e);
+    const int32_t r = in_lds ? r0 + upper_row(srp, 0, span - 1, e) : upper_row(rowptr, r0, r1, e);
     const double *xr = x + (int64_t)colind[e] * ldx;
     const double *dr = dy + (int64_t)r * lddy;
     double s = dr[0] * xr[0];
@@ -701,15 +694,18 @@ void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const
   hip_check(hipGetLastError(), "scatter launch");
 }
 
-void launch_dvals(const int32_t *rowptr, const int32_t *colind, int32_t nrows, int64_t nnz,
-                  const double *dy, int64_t lddy, const double *x, int64_t ldx, int32_t k,
-                  double *dvals, hipStream_t stream) {
+void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *tile_row,
+                  int32_t nrows, int64_t nnz, const double *dy, int64_t lddy, const double *x,
+                  int64_t ldx, int32_t k, double *dvals, hipStream_t stream) {
   if (nnz <= 0) return;
   const int64_t blocks = (nnz + kDvTile - 1) / kDvTile;
   if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
   hipLaunchKernelGGL(dvals_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, rowptr, colind,
-                     nrows, nnz, dy, lddy, x, ldx, k, dvals);
+                     tile_row, nrows, nnz, dy, lddy, x, ldx, k, dvals);
   hip_check(hipGetLastError(), "dvals launch");
 }
+
+int64_t dvals_tiles(int64_t nnz) { return (nnz + kDvTile - 1) / kDvTile; }
+int64_t dvals_tile_size() { return kDvTile; }
 
 }  // namespace wk

@@ -91,6 +91,7 @@ struct wk_engine {
   wk::DevLayout adj;                // layout of A^T
   int32_t *d_rowptr = nullptr;      // CSR pattern for dvals (adjoint engines)
   int32_t *d_colind = nullptr;
+  int32_t *d_tile_row = nullptr;    // dvals tile -> first row
   std::vector<void *> owned;        // every hipMalloc of this engine
   int device = 0;
   std::mutex cfg_mu;                // guards launch_cfg
@@ -127,9 +128,13 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
 void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
                            hipStream_t stream);
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
-void launch_dvals(const int32_t *rowptr, const int32_t *colind, int32_t nrows, int64_t nnz,
-                  const double *dy, int64_t lddy, const double *x, int64_t ldx, int32_t k,
-                  double *dvals, hipStream_t stream);
+void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *tile_row,
+                  int32_t nrows, int64_t nnz, const double *dy, int64_t lddy, const double *x,
+                  int64_t ldx, int32_t k, double *dvals, hipStream_t stream);
+// dvals works on tiles of consecutive entries; tile_row[t] = row of entry
+// t*tile_size, tile_row[tiles] = nrows - 1.
+int64_t dvals_tiles(int64_t nnz);
+int64_t dvals_tile_size();
 
 // The configuration an engine uses for (op, k) when the caller passes none:
 // the tuned one if any, else a choice from the layout.
