@@ -141,6 +141,7 @@ def test_dropin_driver_flow_wpk1():
     dict(rows_per_lane=4, unroll=4), dict(rows_per_lane=4), dict(blocks_per_cu=3),
     dict(rows_per_lane=2, waves_per_block=1, blocks_per_cu=6),
     dict(stage_lds=1), dict(stage_lds=1, unroll=16), dict(stage_lds=1, waves_per_block=1, unroll=16),
+    dict(stage_lds=2), dict(stage_lds=2, unroll=16), dict(stage_lds=2, waves_per_block=2, unroll=16),
 ])
 def test_launch_options_do_not_change_results(cfg):
     say(f"[gpu] launch options {cfg}")

@@ -322,7 +322,7 @@ constexpr int staged_bytes() {
   return U * 64 * (int)(sizeof(typename P::V) + sizeof(int32_t));
 }
 
-template <class P, bool SPLIT, int OUT, int U>
+template <class P, bool SPLIT, int OUT, int U, bool PIPE>
 __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, int32_t nwork,
                                                                   int32_t remap) {
   using V = typename P::V;
@@ -360,23 +360,66 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, in
     P::first(acc, v, xb[(int64_t)c * ldx]);
   }
   uint32_t i = 1;
-  for (; i + U <= m.len; i += U) {
-    const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)i * W);
-    const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)i * W);
+  if constexpr (!PIPE) {
+    for (; i + U <= m.len; i += U) {
+      const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)i * W);
+      const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)i * W);
+      u32x4 tc[NC], tv[NV];
+#pragma unroll
+      for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
+#pragma unroll
+      for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+#pragma unroll
+      for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
+#pragma unroll
+      for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
+      X xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
+#pragma unroll
+      for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+    }
+  } else if (i + U <= m.len) {
+    // Depth-2 pipeline: once a batch sits in LDS its registers are free, so
+    // the next batch's 16-byte loads are issued right after this batch's
+    // gathers and stay in flight through the FMAs.  The loop body is
+    // straight-line and both edges into its header carry the same loads
+    // into the same registers.
     u32x4 tc[NC], tv[NV];
+    auto issue = [&](uint32_t at) {
+      const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)at * W);
+      const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)at * W);
 #pragma unroll
-    for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
+      for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
 #pragma unroll
-    for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+      for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+    };
+    auto stage = [&]() {
 #pragma unroll
-    for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
+      for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
 #pragma unroll
-    for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
+      for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
+    };
+    issue(i);
+    for (; i + 2 * U <= m.len; i += U) {
+      stage();
+      X xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
+      issue(i + U);
+      V v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = lv[u * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < U; u++) P::step(acc, v[u], xv[u]);
+    }
+    stage();
     X xv[U];
 #pragma unroll
     for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
 #pragma unroll
     for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+    i += U;
   }
   for (; i < m.len; i++) {
     const int32_t c = cbase[(int64_t)i * W + lane];
@@ -452,10 +495,36 @@ __global__ __launch_bounds__(256) void dvals_kernel(const int32_t *__restrict__ 
   if (in_lds)
     for (int32_t t = threadIdx.x; t < span; t += blockDim.x) srp[t] = rowptr[r0 + t];
   __syncthreads();
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const int32_t r = in_lds ? r0 + upper_row(srp, 0, span - 1, e) : upper_row(rowptr, r0, r1, e);
-    const double *xr = x + (int64_t)colind[e] * ldx;
-    const double *dr = dy + (int64_t)r * lddy;
+  // all entries of a thread at once: rows, then indices, then gathers, so
+  // their memory latencies overlap instead of adding up
+  constexpr int PER = kDvTile / 256;
+  int32_t r[PER], c[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int64_t e = e0 + threadIdx.x + (int64_t)q * blockDim.x;
+    r[q] = e < e1 ? (in_lds ? r0 + upper_row(srp, 0, span - 1, e) : upper_row(rowptr, r0, r1, e)) : r0;
+    c[q] = e < e1 ? colind[e] : 0;
+  }
+  if (k == 1) {
+    double xv[PER], dvv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      xv[q] = x[(int64_t)c[q] * ldx];
+      dvv[q] = dy[(int64_t)r[q] * lddy];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int64_t e = e0 + threadIdx.x + (int64_t)q * blockDim.x;
+      if (e < e1) dv[e] = dvv[q] * xv[q];
+    }
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int64_t e = e0 + threadIdx.x + (int64_t)q * blockDim.x;
+    if (e >= e1) continue;
+    const double *xr = x + (int64_t)c[q] * ldx;
+    const double *dr = dy + (int64_t)r[q] * lddy;
     double s = dr[0] * xr[0];
     for (int32_t j = 1; j < k; j++) s = __builtin_fma(dr[j], xr[j], s);
     dv[e] = s;
@@ -534,31 +603,36 @@ LaunchFn pick(int prec, int W, bool split, int out, int kt, int u, bool nt, int 
 
 using StagedFn = void (*)(const SpmvArgs &, dim3, dim3, size_t, int32_t, int32_t, hipStream_t);
 
-template <class P, bool SPLIT, int OUT, int U>
+template <class P, bool SPLIT, int OUT, int U, bool PIPE>
 void launch_staged(const SpmvArgs &a, dim3 grid, dim3 block, size_t lds, int32_t nwork,
                    int32_t remap, hipStream_t s) {
-  hipLaunchKernelGGL((spmv_staged_kernel<P, SPLIT, OUT, U>), grid, block, lds, s, a, nwork, remap);
+  hipLaunchKernelGGL((spmv_staged_kernel<P, SPLIT, OUT, U, PIPE>), grid, block, lds, s, a, nwork, remap);
+}
+
+template <class P, bool SPLIT, int OUT>
+StagedFn pick_staged_u(int u, bool pipe) {
+  if (u == 16) return pipe ? &launch_staged<P, SPLIT, OUT, 16, true> : &launch_staged<P, SPLIT, OUT, 16, false>;
+  return pipe ? &launch_staged<P, SPLIT, OUT, 8, true> : &launch_staged<P, SPLIT, OUT, 8, false>;
 }
 
 template <class P, bool SPLIT>
-StagedFn pick_staged_out(int out, int u) {
-  if (out == OUT_RUN) return u == 16 ? &launch_staged<P, SPLIT, OUT_RUN, 16> : &launch_staged<P, SPLIT, OUT_RUN, 8>;
-  if (out == OUT_RUNX) return u == 16 ? &launch_staged<P, SPLIT, OUT_RUNX, 16> : &launch_staged<P, SPLIT, OUT_RUNX, 8>;
-  if constexpr (std::is_same_v<P, PF64>)
-    return u == 16 ? &launch_staged<P, SPLIT, OUT_RESID, 16> : &launch_staged<P, SPLIT, OUT_RESID, 8>;
+StagedFn pick_staged_out(int out, int u, bool pipe) {
+  if (out == OUT_RUN) return pick_staged_u<P, SPLIT, OUT_RUN>(u, pipe);
+  if (out == OUT_RUNX) return pick_staged_u<P, SPLIT, OUT_RUNX>(u, pipe);
+  if constexpr (std::is_same_v<P, PF64>) return pick_staged_u<P, SPLIT, OUT_RESID>(u, pipe);
   return nullptr;
 }
 
 template <class P>
-StagedFn pick_staged_split(bool split, int out, int u) {
-  return split ? pick_staged_out<P, true>(out, u) : pick_staged_out<P, false>(out, u);
+StagedFn pick_staged_split(bool split, int out, int u, bool pipe) {
+  return split ? pick_staged_out<P, true>(out, u, pipe) : pick_staged_out<P, false>(out, u, pipe);
 }
 
-StagedFn pick_staged(int prec, bool split, int out, int u) {
+StagedFn pick_staged(int prec, bool split, int out, int u, bool pipe) {
   if (u != 8 && u != 16) return nullptr;
-  if (prec == 0) return pick_staged_split<PF64>(split, out, u);
-  if (prec == 1) return pick_staged_split<PF32C>(split, out, u);
-  return pick_staged_split<PF32D>(split, out, u);
+  if (prec == 0) return pick_staged_split<PF64>(split, out, u, pipe);
+  if (prec == 1) return pick_staged_split<PF32C>(split, out, u, pipe);
+  return pick_staged_split<PF32D>(split, out, u, pipe);
 }
 
 size_t staged_lds_per_wave(int prec, int u) {
@@ -623,7 +697,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     // rows per lane applies to single right-hand-side tiles of W = 64 layouts
     const int r = (kt == 1 && D.W == 64) ? rpl : 1;
     if (cfg.stage_lds && kt == 1 && D.W == 64 && r == 1) {
-      StagedFn sf = pick_staged(prec, split, out, cfg.unroll);
+      StagedFn sf = pick_staged(prec, split, out, cfg.unroll, cfg.stage_lds == 2);
       if (!sf) fail(WK_ERR_ARG, "LDS staging needs unroll 8 or 16");
       const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
       const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);

@@ -43,8 +43,10 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
           c.stage_lds = 0;
           out.push_back(c);
           if (D.W == 64 && (size_t)wpb * u * 64 * (vsz + 4) <= 65536) {
-            c.stage_lds = 1;
-            out.push_back(c);
+            for (int st : {1, 2}) {
+              c.stage_lds = st;
+              out.push_back(c);
+            }
             c.stage_lds = 0;
           }
         }
