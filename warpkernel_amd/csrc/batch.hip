@@ -68,7 +68,7 @@ struct BF64 {
   struct Acc {
     double s;
   };
-  __device__ static void first(Acc &a, V v, X x) { a.s = v * x; }
+  __device__ static void init(Acc &a) { a.s = -0.0; }  // fma(v, x, -0) == v*x bit for bit
   __device__ static void step(Acc &a, V v, X x) { a.s = __builtin_fma(v, x, a.s); }
   __device__ static void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
   __device__ static Acc shfl_down(const Acc &a, int o) { return Acc{__shfl_down(a.s, o, 64)}; }
@@ -87,9 +87,9 @@ struct BF32C {
     float ap = s - bp;
     t = (a - ap) + (b - bp);
   }
-  __device__ static void first(Acc &a, V v, X x) {
-    a.s = v * x;
-    a.c = __builtin_fmaf(v, x, -a.s);
+  __device__ static void init(Acc &a) {
+    a.s = -0.0f;
+    a.c = 0.0f;
   }
   __device__ static void step(Acc &a, V v, X x) {
     float p = v * x, e = __builtin_fmaf(v, x, -p), s, t;
@@ -115,7 +115,7 @@ struct BF32D {
   struct Acc {
     double s;
   };
-  __device__ static void first(Acc &a, V v, X x) { a.s = (double)v * (double)x; }
+  __device__ static void init(Acc &a) { a.s = -0.0; }
   __device__ static void step(Acc &a, V v, X x) { a.s = __builtin_fma((double)v, (double)x, a.s); }
   __device__ static void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
   __device__ static Acc shfl_down(const Acc &a, int o) { return Acc{__shfl_down(a.s, o, 64)}; }
@@ -138,13 +138,9 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   const int32_t *cp = d.cols + base + lane;
   const X *xb = static_cast<const X *>(d.x) + j0;
   Acc acc[KT];
-  {
-    const int32_t c = cp[0];
-    const V v = __builtin_nontemporal_load(vp);
 #pragma unroll
-    for (int j = 0; j < KT; j++) P::first(acc[j], v, xb[(int64_t)c * d.ldx + j]);
-  }
-  uint32_t i = 1;
+  for (int j = 0; j < KT; j++) P::init(acc[j]);
+  uint32_t i = 0;
   for (; i + U <= len; i += U) {
     int32_t c[U];
     V v[U];
@@ -163,11 +159,26 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
 #pragma unroll
       for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
   }
-  for (; i < len; i++) {
-    const int32_t c = cp[(int64_t)i * 64];
-    const V v = __builtin_nontemporal_load(vp + (int64_t)i * 64);
+  if (i < len) {  // last partial batch, loads issued together (uniform guards)
+    int32_t c[U];
+    V v[U];
 #pragma unroll
-    for (int j = 0; j < KT; j++) P::step(acc[j], v, xb[(int64_t)c * d.ldx + j]);
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) {
+        c[u] = cp[(int64_t)(i + u) * 64];
+        v[u] = __builtin_nontemporal_load(vp + (int64_t)(i + u) * 64);
+      }
+    X xv[U][KT];
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len)
+#pragma unroll
+        for (int j = 0; j < KT; j++) xv[u][j] = xb[(int64_t)c[u] * d.ldx + j];
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len)
+#pragma unroll
+        for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
   }
   for (uint32_t o = 1; o < tpr; o <<= 1) {
 #pragma unroll

@@ -46,6 +46,10 @@ struct PF64 {
   struct Acc {
     double s;
   };
+  // fma(v, x, -0) is the rounded product v*x bit for bit (signed zeros
+  // included), so starting at -0 and adding every slot with fma reproduces the
+  // reference's "first product, then fma" order
+  __device__ static __forceinline__ void init(Acc &a) { a.s = -0.0; }
   __device__ static __forceinline__ void first(Acc &a, V v, X x) { a.s = v * x; }
   __device__ static __forceinline__ void step(Acc &a, V v, X x) { a.s = __builtin_fma(v, x, a.s); }
   __device__ static __forceinline__ void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
@@ -63,6 +67,10 @@ struct PF32C {
   struct Acc {
     float s, c;
   };
+  __device__ static __forceinline__ void init(Acc &a) {
+    a.s = -0.0f;
+    a.c = 0.0f;
+  }
   __device__ static __forceinline__ void first(Acc &a, V v, X x) {
     a.s = v * x;
     a.c = __builtin_fmaf(v, x, -a.s);
@@ -103,6 +111,7 @@ struct PF32D {
   struct Acc {
     double s;
   };
+  __device__ static __forceinline__ void init(Acc &a) { a.s = -0.0; }
   __device__ static __forceinline__ void first(Acc &a, V v, X x) { a.s = (double)v * (double)x; }
   __device__ static __forceinline__ void step(Acc &a, V v, X x) {
     a.s = __builtin_fma((double)v, (double)x, a.s);
@@ -185,6 +194,39 @@ __device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
   return i;
 }
 
+// The last, partial batch of one group: steps [i, end) with end - i < U, all
+// loads issued before the first wait (the guards are wave-uniform for W = 64).
+template <class P, int W, int KT, int U, bool NT>
+__device__ __forceinline__ void stream_tail(typename P::Acc (&acc)[KT], const typename P::V *vp,
+                                            const int32_t *cp, const typename P::X *xb,
+                                            int64_t ldx, uint32_t i, uint32_t end) {
+  using V = typename P::V;
+  using X = typename P::X;
+  if (i >= end) return;
+  int32_t c[U];
+  V v[U];
+#pragma unroll
+  for (int u = 0; u < U - 1; u++)
+    if (i + u < end) {
+      c[u] = cp[(int64_t)(i + u) * W];
+      v[u] = ld_stream<NT>(vp + (int64_t)(i + u) * W);
+    }
+  X xv[U][KT];
+#pragma unroll
+  for (int u = 0; u < U - 1; u++)
+    if (i + u < end) {
+      const X *xr = xb + (int64_t)c[u] * ldx;
+#pragma unroll
+      for (int j = 0; j < KT; j++) xv[u][j] = xr[j];
+    }
+#pragma unroll
+  for (int u = 0; u < U - 1; u++)
+    if (i + u < end) {
+#pragma unroll
+      for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
+    }
+}
+
 // grid.x = nwork + fill blocks.  Work block -> (group block, k-tile), k-tile
 // fastest, so blocks running together stream the same groups.  With W = 64 a
 // wave owns R consecutive groups ("rows per lane" = R); with W = 32 it owns
@@ -239,25 +281,17 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_kernel(SpmvArgs a, int32_t n
   const X *__restrict__ xb = static_cast<const X *>(a.x) + j0;
   const int64_t ldx = a.ldx;
 
-  // first slot of each row: rounded product (warpkernel.hpp:68)
+  // every slot, the first included, by fma onto -0 (see PF64::init)
   Acc acc[R][KT];
   uint32_t common = 0xffffffffu;
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    if (m[r].len > 0) {
-      const int32_t c = *cp[r];
-      const V v = ld_stream<NT>(vp[r]);
-      const X *xr = xb + (int64_t)c * ldx;
 #pragma unroll
-      for (int j = 0; j < KT; j++) P::first(acc[r][j], v, xr[j]);
-    } else {
-#pragma unroll
-      for (int j = 0; j < KT; j++) P::first(acc[r][j], V(0), X(0));
-    }
+    for (int j = 0; j < KT; j++) P::init(acc[r][j]);
     common = m[r].len < common ? m[r].len : common;
   }
   // all R groups together up to the shortest, then each group alone
-  uint32_t i = stream_steps<P, W, KT, U, NT, R>(acc, vpc, cpc, xb, ldx, 1u, common);
+  uint32_t i = stream_steps<P, W, KT, U, NT, R>(acc, vpc, cpc, xb, ldx, 0u, common);
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const V *const vr[1] = {vp[r]};
@@ -265,16 +299,10 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_kernel(SpmvArgs a, int32_t n
     Acc ar[1][KT];
 #pragma unroll
     for (int j = 0; j < KT; j++) ar[0][j] = acc[r][j];
-    uint32_t ii = stream_steps<P, W, KT, U, NT, 1>(ar, vr, cr, xb, ldx, i, m[r].len);
+    const uint32_t ii = stream_steps<P, W, KT, U, NT, 1>(ar, vr, cr, xb, ldx, i, m[r].len);
+    stream_tail<P, W, KT, U, NT>(ar[0], vp[r], cp[r], xb, ldx, ii, m[r].len);
 #pragma unroll
     for (int j = 0; j < KT; j++) acc[r][j] = ar[0][j];
-    for (; ii < m[r].len; ii++) {
-      const int32_t c = cp[r][(int64_t)ii * W];
-      const V v = ld_stream<NT>(vp[r] + (int64_t)ii * W);
-      const X *xr = xb + (int64_t)c * ldx;
-#pragma unroll
-      for (int j = 0; j < KT; j++) P::step(acc[r][j], v, xr[j]);
-    }
   }
 
 #pragma unroll
@@ -354,12 +382,8 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, in
   const int64_t ldx = a.ldx;
 
   Acc acc;
-  {
-    const int32_t c = cbase[lane];
-    const V v = __builtin_nontemporal_load(vbase + lane);
-    P::first(acc, v, xb[(int64_t)c * ldx]);
-  }
-  uint32_t i = 1;
+  P::init(acc);  // every slot by fma onto -0 (see PF64::init)
+  uint32_t i = 0;
   if constexpr (!PIPE) {
     for (; i + U <= m.len; i += U) {
       const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)i * W);
@@ -421,10 +445,10 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, in
     for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
     i += U;
   }
-  for (; i < m.len; i++) {
-    const int32_t c = cbase[(int64_t)i * W + lane];
-    const V v = __builtin_nontemporal_load(vbase + (int64_t)i * W + lane);
-    P::step(acc, v, xb[(int64_t)c * ldx]);
+  {
+    Acc ar[1] = {acc};
+    stream_tail<P, W, 1, U, true>(ar, vbase + lane, cbase + lane, xb, ldx, i, m.len);
+    acc = ar[0];
   }
   if constexpr (SPLIT) {
     for (uint32_t o = 1; o < m.tpr; o <<= 1) {
@@ -459,23 +483,11 @@ __global__ void scatter_kernel(const T *__restrict__ src, const uint32_t *__rest
 
 // dvals[t] = sum_j dy[row_t][j] * x[col_t][j], CSR order.  Each block takes a
 // tile of kDvTile consecutive entries; tile_row[b] (built with the engine) is
-// the row holding the tile's first entry, so the block loads the rowptr slice
-// it spans into LDS and each entry finds its row by a short binary search
-// there.  Column reads and dvals writes are coalesced.
+// the row holding the tile's first entry.  The block's threads walk the rows
+// the tile spans and stamp each entry slot of an LDS array with its row, so
+// every entry then finds its row with one conflict-free LDS read.  Column
+// reads and dvals writes are coalesced.
 constexpr int kDvTile = 2048;
-constexpr int kDvRows = 2048;
-
-__device__ __forceinline__ int32_t upper_row(const int32_t *rp, int32_t lo, int32_t hi, int64_t e) {
-  // largest r in [lo, hi] with rp[r] <= e
-  while (lo < hi) {
-    int32_t mid = lo + ((hi - lo + 1) >> 1);
-    if ((int64_t)rp[mid] <= e)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
 
 __global__ __launch_bounds__(256) void dvals_kernel(const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ colind,
@@ -484,25 +496,26 @@ __global__ __launch_bounds__(256) void dvals_kernel(const int32_t *__restrict__ 
                                                     const double *__restrict__ dy, int64_t lddy,
                                                     const double *__restrict__ x, int64_t ldx,
                                                     int32_t k, double *__restrict__ dv) {
-  __shared__ int32_t srp[kDvRows + 1];
+  __shared__ int32_t srow[kDvTile];
   const int64_t e0 = (int64_t)blockIdx.x * kDvTile;
   const int64_t e1 = e0 + kDvTile < nnz ? e0 + kDvTile : nnz;
   // rows of the tile lie in [r0, r1]: the next tile starts in row r1 or later
   const int32_t r0 = tile_row[blockIdx.x];
   const int32_t r1 = tile_row[blockIdx.x + 1];
-  const int32_t span = r1 - r0 + 1;
-  const bool in_lds = span <= kDvRows;
-  if (in_lds)
-    for (int32_t t = threadIdx.x; t < span; t += blockDim.x) srp[t] = rowptr[r0 + t];
+  for (int32_t r = r0 + (int32_t)threadIdx.x; r <= r1; r += (int32_t)blockDim.x) {
+    const int64_t a = rowptr[r] > e0 ? rowptr[r] : e0;
+    const int64_t b = rowptr[r + 1] < e1 ? rowptr[r + 1] : e1;
+    for (int64_t e = a; e < b; e++) srow[e - e0] = r;
+  }
   __syncthreads();
-  // all entries of a thread at once: rows, then indices, then gathers, so
-  // their memory latencies overlap instead of adding up
+  // all entries of a thread at once: rows, indices, then gathers, so their
+  // memory latencies overlap instead of adding up
   constexpr int PER = kDvTile / 256;
   int32_t r[PER], c[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int64_t e = e0 + threadIdx.x + (int64_t)q * blockDim.x;
-    r[q] = e < e1 ? (in_lds ? r0 + upper_row(srp, 0, span - 1, e) : upper_row(rowptr, r0, r1, e)) : r0;
+    r[q] = e < e1 ? srow[e - e0] : r0;
     c[q] = e < e1 ? colind[e] : 0;
   }
   if (k == 1) {

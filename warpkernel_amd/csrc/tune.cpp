@@ -75,10 +75,11 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t k) {
   wk_launch_default(&c);
   const double mean_len =
       D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * D.W) : 0.0;
-  if (k == 1 && D.W == 64 && mean_len >= 40.0) {
-    // long rows: 16-byte batch copies through LDS (measured on hex elasticity)
-    c.stage_lds = 1;
-    c.unroll = 16;
+  if (k == 1 && D.W == 64 && mean_len >= 12.0) {
+    // 16-byte batch copies through LDS, next batch in flight during the FMAs
+    // (measured best on hex elasticity and hex diffusion)
+    c.stage_lds = 2;
+    c.unroll = 8;
     c.waves_per_block = 4;
     c.xcd_remap = 0;
   } else {
