@@ -160,9 +160,10 @@ typedef struct wk_launch {
   int32_t blocks_per_cu;   /* 0: no cap; else resident blocks per CU are capped
                               by reserving LDS (occupancy control) */
   int32_t stage_lds;       /* 1: copy value/index batches through LDS with
-                              16-byte loads (k_tile 1, group width 64, unroll 8/16);
-                              2: the same with the next batch's loads issued
-                              before the current batch's FMAs */
+                              16-byte loads; 2: the same with the next batch's
+                              loads issued before the current batch's FMAs.
+                              Applies to k_tile 1, group width 64, rows_per_lane
+                              1, unroll 8/16, nontemporal 1; otherwise ignored */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);

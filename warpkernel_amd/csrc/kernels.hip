@@ -709,9 +709,13 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const int32_t ntiles = rem / kt;
     // rows per lane applies to single right-hand-side tiles of W = 64 layouts
     const int r = (kt == 1 && D.W == 64) ? rpl : 1;
-    if (cfg.stage_lds && kt == 1 && D.W == 64 && r == 1) {
+    // LDS staging where it is offered (one right-hand side, W = 64, one group
+    // per wave, unroll 8 or 16, nontemporal values); the direct kernel otherwise
+    const bool staged = cfg.stage_lds && kt == 1 && D.W == 64 && r == 1 && cfg.nontemporal &&
+                        (cfg.unroll == 8 || cfg.unroll == 16);
+    if (staged) {
       StagedFn sf = pick_staged(prec, split, out, cfg.unroll, cfg.stage_lds == 2);
-      if (!sf) fail(WK_ERR_ARG, "LDS staging needs unroll 8 or 16");
+      if (!sf) fail(WK_ERR_ARG, "unsupported staged configuration");
       const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
       const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
       const size_t need = staged_lds_per_wave(prec, cfg.unroll) * (size_t)wpb;
