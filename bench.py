@@ -107,12 +107,11 @@ def cpu_baseline(m, x, budget_s):
     }
 
 
-def extras(torch, dev, steps, warmup):
+def extra_multi_rhs(torch, dev, steps, warmup, tune=True):
+    """FP32-compensated multi-RHS vs plain FP64 (hex elasticity 32^3)."""
     from oracle import pyoracle as O
-    from warpkernel_amd import engine, fe, structure, structure2
+    from warpkernel_amd import engine, fe, structure
     out = {}
-
-    # FP32-compensated multi-RHS vs plain FP64 (hex elasticity 32^3)
     m = fe.hex_elastic_3d(32)
     s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
     e64 = engine(s, m.vals, m.colind)
@@ -126,13 +125,16 @@ def extras(torch, dev, steps, warmup):
         dx64, dx32 = dev(x), dev(xq)
         y64 = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
         y32 = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
-        e64.tune(dx64, y64, reps=5)
-        e32.tune(dx32, y32, reps=5)
+        if tune:
+            e64.tune(dx64, y64, reps=5)
+            e32.tune(dx32, y32, reps=5)
         _, ms64 = time_launches(torch, lambda: e64.launch(dx64, y64), steps, warmup)
         _, ms32 = time_launches(torch, lambda: e32.launch(dx32, y32), steps, warmup)
-        e32.tune(dx32, y32, reps=5, base=dict(accumulate=1))
+        if tune:
+            e32.tune(dx32, y32, reps=5, base=dict(accumulate=1))
         _, ms32d = time_launches(torch, lambda: e32.launch(dx32, y32, cfg=dict(accumulate=1)), steps, warmup)
-        e32.tune(dx32, y32, reps=5, base=dict(accumulate=0))
+        if tune:
+            e32.tune(dx32, y32, reps=5, base=dict(accumulate=0))
         e32.launch(dx32, y32)
         torch.cuda.synchronize()
         ref_q = O.csr_spmv(m.rowptr, m.colind, vq.astype(np.float64), xq.astype(np.float64))
@@ -155,8 +157,13 @@ def extras(torch, dev, steps, warmup):
         })
     out["multi_rhs_hex_elastic_3d_32"] = rows
     del e64, e32
+    return out
 
-    # forward + adjoint (hex diffusion 48^3): y = A x, dx = A^T dy, dvals
+
+def extra_fwd_adjoint(torch, dev, steps, warmup, tune=True):
+    """Forward + adjoint (hex diffusion 48^3): y = A x, dx = A^T dy, dvals."""
+    from warpkernel_amd import engine, fe, structure
+    out = {}
     m = fe.hex_laplace_3d(48)
     s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
     eng = engine(s, m.vals, m.colind, adjoint=True)
@@ -167,8 +174,9 @@ def extras(torch, dev, steps, warmup):
         y = torch.empty((m.nrows, k), dtype=torch.float64, device="cuda")
         dxv = torch.empty((m.ncols, k), dtype=torch.float64, device="cuda")
         dv = torch.empty(m.nnz, dtype=torch.float64, device="cuda")
-        eng.tune(x, y, reps=5)
-        eng.tune(dyv, dxv, op=2, reps=5)
+        if tune:
+            eng.tune(x, y, reps=5)
+            eng.tune(dyv, dxv, op=2, reps=5)
         _, f = time_launches(torch, lambda: eng.launch(x, y), steps, warmup)
         _, a = time_launches(torch, lambda: eng.run_adjoint(dyv, dxv), steps, warmup)
         _, d = time_launches(torch, lambda: eng.dvals(dyv, x, dv), steps, warmup)
@@ -177,7 +185,13 @@ def extras(torch, dev, steps, warmup):
                        "fwd_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (f * 1e-3) / 1e9, 1)}
     out["fwd_adjoint_hex_laplace_3d_48"] = fa
     del eng
+    return out
 
+
+def extra_batch(torch, dev, steps, warmup, tune=True):
+    """64 mixed FE matrices in one persistent launch."""
+    from warpkernel_amd import engine, fe, structure
+    out = {}
     # 64 mixed FE matrices (2D Q1 / 3D hex elasticity, 2k-250k rows): one
     # persistent launch vs one launch per matrix vs a replayed HIP graph
     from warpkernel_amd import batch, graph
@@ -211,6 +225,9 @@ def extras(torch, dev, steps, warmup):
     return out
 
 
+EXTRAS = {"multi_rhs": extra_multi_rhs, "fwd_adjoint": extra_fwd_adjoint, "batch": extra_batch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (layout default)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--only", choices=sorted(EXTRAS), default=None,
+                    help="run one extra section alone (no headline), for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -235,6 +254,11 @@ def main():
 
     def dev(a):
         return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+    if args.only:
+        res = EXTRAS[args.only](torch, dev, max(20, args.steps // 5), 10, tune=not args.no_tune)
+        print(json.dumps({"section": args.only, "tuned": not args.no_tune, **res}), flush=True)
+        return
 
     print("[bench] building hex_elastic_3d_64", file=sys.stderr, flush=True)
     m = fe.hex_elastic_3d(64)
@@ -304,7 +328,8 @@ def main():
         ex[f"wpk2_t{args.threshold}_GBs"] = round(nbytes / (ms2 * 1e-3) / 1e9, 2)
         del e2
         print("[bench] extras", file=sys.stderr, flush=True)
-        ex.update(extras(torch, dev, max(20, args.steps // 5), 10))
+        for fn in EXTRAS.values():
+            ex.update(fn(torch, dev, max(20, args.steps // 5), 10, tune=not args.no_tune))
         line["extras"] = ex
     else:
         del eng

@@ -234,9 +234,11 @@ int wk_engine_set_launch(wk_engine *engine, int32_t op, int32_t k, const wk_laun
  * value type, use group width 64 and outlive the batch. */
 typedef struct wk_batch wk_batch;
 
-/* waves_per_cu: persistent waves per compute unit (0: 8). */
+/* waves_per_cu: persistent waves per compute unit (0: 8).  stage_lds: as in
+ * wk_launch, for batches with one right-hand side (0 direct loads, 1 staged,
+ * 2 staged with the next batch in flight). */
 int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
-                    wk_batch **out);
+                    int32_t stage_lds, wk_batch **out);
 
 /* Binds the vectors of every matrix: x_devs[i] ([ncols_i][ldx[i]]) and y_devs[i]
  * ([nrows_i][ldy[i]]) for k right-hand sides (1..32).  Host arrays of device

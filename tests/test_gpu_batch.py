@@ -49,17 +49,17 @@ def make_engines(ms, dtype="f64"):
     return engs
 
 
-@pytest.mark.parametrize("k", [1, 4, 3])
-def test_batch_of_64_matrices_bitexact(k):
+@pytest.mark.parametrize("k,stage", [(1, 0), (1, 1), (1, 2), (4, 2), (3, 0)])
+def test_batch_of_64_matrices_bitexact(k, stage):
     ms = mix()
-    say(f"[gpu] batch of {len(ms)} matrices, {sum(m.nnz for m in ms)} nnz, k={k}")
+    say(f"[gpu] batch of {len(ms)} matrices, {sum(m.nnz for m in ms)} nnz, k={k}, stage={stage}")
     engs = make_engines(ms)
     xs_h = [fe.uniform_x(m.ncols, k, seed=7000 + i).reshape((m.ncols,) if k == 1 else (m.ncols, k))
             for i, m in enumerate(ms)]
     xs = [dev(x) for x in xs_h]
     ys = [torch.full((m.nrows,) if k == 1 else (m.nrows, k), float("nan"), dtype=torch.float64,
                      device="cuda") for m in ms]
-    b = batch(engs)
+    b = batch(engs, stage_lds=stage)
     b.bind(xs, ys)
     b.run()
     torch.cuda.synchronize()

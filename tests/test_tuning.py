@@ -1,0 +1,36 @@
+"""Layout tuner: the reference's threshold sweep (CPU) and an end-to-end run (GPU)."""
+import numpy as np
+import pytest
+
+from warpkernel_amd import fe
+from warpkernel_amd.tuning import thresholds
+
+
+def test_threshold_sweep_matches_reference_driver():
+    # wpk2.cu:156-157: inc = (max-min)*0.1 if positive else 1; th = min..max step inc
+    assert thresholds(24, 81) == list(range(24, 82, 5))
+    assert thresholds(9, 9) == [9]
+    assert thresholds(3, 10) == list(range(3, 11, 1))
+    assert thresholds(1, 400) == list(range(1, 401, 39))
+
+
+@pytest.mark.gpu
+def test_tune_layout_returns_a_working_engine():
+    import torch
+    from oracle import pyoracle as O
+    from warpkernel_amd.tuning import tune_layout
+    m = fe.hex_laplace_3d(24)
+    x = torch.from_numpy(fe.uniform_x(m.ncols, seed=8)).cuda()
+    y = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+    best = tune_layout(m.rowptr, m.colind, m.vals, x, y, reps=3, threshold_list=[4, 7, 14])
+    assert best.variant in (1, 2) and best.ms > 0 and len(best.table) == 4
+    y.fill_(float("nan"))
+    best.engine.launch(x, y)
+    torch.cuda.synchronize()
+    ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x.cpu().numpy())
+    if best.variant == 1:
+        assert np.array_equal(y.cpu().numpy(), ref)
+    else:
+        L = O.scan2(m.rowptr, best.threshold, 64, 32)
+        pv, pc = O.process(L, m.vals, m.colind)
+        assert np.array_equal(y.cpu().numpy(), O.wpk_spmv(L, pv, pc, x.cpu().numpy()))

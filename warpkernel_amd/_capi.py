@@ -81,7 +81,7 @@ SIGNATURES = {
                           C.POINTER(C.c_float)]),
     "wk_engine_get_launch": (C.c_int, [P, I32, I32, C.POINTER(Launch)]),
     "wk_engine_set_launch": (C.c_int, [P, I32, I32, C.POINTER(Launch)]),
-    "wk_batch_create": (C.c_int, [C.POINTER(C.c_void_p), I32, I32, PP]),
+    "wk_batch_create": (C.c_int, [C.POINTER(C.c_void_p), I32, I32, I32, PP]),
     "wk_batch_bind": (C.c_int, [P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), I32,
                                 C.POINTER(I64), C.POINTER(I64)]),
     "wk_batch_run": (C.c_int, [P, P]),

@@ -361,11 +361,11 @@ class batch:
 
     xs / ys: lists of device tensors, [ncols_i] / [nrows_i] or [n, k]."""
 
-    def __init__(self, engines, waves_per_cu: int = 0):
+    def __init__(self, engines, waves_per_cu: int = 0, stage_lds: int = 2):
         self.engines = list(engines)  # keep them alive
         arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
         h = C.c_void_p()
-        check(lib().wk_batch_create(arr, len(self.engines), int(waves_per_cu), C.byref(h)))
+        check(lib().wk_batch_create(arr, len(self.engines), int(waves_per_cu), int(stage_lds), C.byref(h)))
         self._h = h
 
     def bind(self, xs, ys):

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 
+#include "spmv_device.hpp"
 #include "wk_internal.hpp"
 
 namespace wk {
@@ -44,6 +45,7 @@ struct wk_batch {
   int32_t nwaves = 0;
   int32_t k = 0;
   int32_t kt = 1;
+  int32_t stage = 2;
   std::vector<wk::BatchDesc> desc;
   wk::BatchDesc *d_desc = nullptr;
   wk::BatchItem *d_items = nullptr;
@@ -59,74 +61,13 @@ struct wk_graph {
 namespace wk {
 namespace {
 
-// The per-precision arithmetic of kernels.hip, repeated here for this
-// translation unit (device functions are not shared across objects without
-// relocatable device code).
-struct BF64 {
-  using V = double;
-  using X = double;
-  struct Acc {
-    double s;
-  };
-  __device__ static void init(Acc &a) { a.s = -0.0; }  // fma(v, x, -0) == v*x bit for bit
-  __device__ static void step(Acc &a, V v, X x) { a.s = __builtin_fma(v, x, a.s); }
-  __device__ static void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
-  __device__ static Acc shfl_down(const Acc &a, int o) { return Acc{__shfl_down(a.s, o, 64)}; }
-  __device__ static double result(const Acc &a) { return a.s; }
-};
-
-struct BF32C {
-  using V = float;
-  using X = float;
-  struct Acc {
-    float s, c;
-  };
-  __device__ static void two_sum(float a, float b, float &s, float &t) {
-    s = a + b;
-    float bp = s - a;
-    float ap = s - bp;
-    t = (a - ap) + (b - bp);
-  }
-  __device__ static void init(Acc &a) {
-    a.s = -0.0f;
-    a.c = 0.0f;
-  }
-  __device__ static void step(Acc &a, V v, X x) {
-    float p = v * x, e = __builtin_fmaf(v, x, -p), s, t;
-    two_sum(a.s, p, s, t);
-    a.s = s;
-    a.c = a.c + (t + e);
-  }
-  __device__ static void add(Acc &a, const Acc &b) {
-    float s, t;
-    two_sum(a.s, b.s, s, t);
-    a.s = s;
-    a.c = (a.c + b.c) + t;
-  }
-  __device__ static Acc shfl_down(const Acc &a, int o) {
-    return Acc{__shfl_down(a.s, o, 64), __shfl_down(a.c, o, 64)};
-  }
-  __device__ static double result(const Acc &a) { return (double)a.s + (double)a.c; }
-};
-
-struct BF32D {
-  using V = float;
-  using X = float;
-  struct Acc {
-    double s;
-  };
-  __device__ static void init(Acc &a) { a.s = -0.0; }
-  __device__ static void step(Acc &a, V v, X x) { a.s = __builtin_fma((double)v, (double)x, a.s); }
-  __device__ static void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
-  __device__ static Acc shfl_down(const Acc &a, int o) { return Acc{__shfl_down(a.s, o, 64)}; }
-  __device__ static double result(const Acc &a) { return a.s; }
-};
-
 // One group of one matrix for right-hand sides [j0, j0+KT): the summation
-// order of warpKernel / warpKernel2 (first product, then fma per slot; the
-// pairwise lane tree for split rows).
-template <class P, int KT, int U>
-__device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32_t j0, int32_t lane) {
+// order of warpKernel / warpKernel2 (every slot by fma onto -0, i.e. first
+// product then fma; the pairwise lane tree for split rows).  With STAGE > 0
+// and KT = 1 the group streams through the wave's LDS slice (16-byte loads).
+template <class P, int KT, int U, int STAGE>
+__device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32_t j0, int32_t lane,
+                                            char *slice) {
   using V = typename P::V;
   using X = typename P::X;
   using Acc = typename P::Acc;
@@ -134,56 +75,30 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   const uint32_t len = d.max_nz_rows[g];
   const uint32_t tpr = d.split ? d.tpr[g] : 1u;
   const uint32_t row0 = d.split ? d.row_off[g] : (uint32_t)g * 64u;
-  const V *vp = static_cast<const V *>(d.vals) + base + lane;
-  const int32_t *cp = d.cols + base + lane;
+  const V *vbase = static_cast<const V *>(d.vals) + base;
+  const int32_t *cbase = d.cols + base;
   const X *xb = static_cast<const X *>(d.x) + j0;
   Acc acc[KT];
+  if constexpr (STAGE > 0 && KT == 1) {
+    acc[0] = staged_group<P, U, STAGE == 2>(vbase, cbase, xb, d.ldx, len, slice, lane);
+  } else {
+    (void)slice;
 #pragma unroll
-  for (int j = 0; j < KT; j++) P::init(acc[j]);
-  uint32_t i = 0;
-  for (; i + U <= len; i += U) {
-    int32_t c[U];
-    V v[U];
+    for (int j = 0; j < KT; j++) P::init(acc[j]);
+    const V *const vp[1] = {vbase + lane};
+    const int32_t *const cp[1] = {cbase + lane};
+    Acc ar[1][KT];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      c[u] = cp[(int64_t)(i + u) * 64];
-      v[u] = __builtin_nontemporal_load(vp + (int64_t)(i + u) * 64);
-    }
-    X xv[U][KT];
+    for (int j = 0; j < KT; j++) ar[0][j] = acc[j];
+    const uint32_t i = stream_steps<P, 64, KT, U, true, 1>(ar, vp, cp, xb, d.ldx, 0u, len);
+    stream_tail<P, 64, KT, U, true>(ar[0], vp[0], cp[0], xb, d.ldx, i, len);
 #pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-      for (int j = 0; j < KT; j++) xv[u][j] = xb[(int64_t)c[u] * d.ldx + j];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-      for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
-  }
-  if (i < len) {  // last partial batch, loads issued together (uniform guards)
-    int32_t c[U];
-    V v[U];
-#pragma unroll
-    for (int u = 0; u < U - 1; u++)
-      if (i + u < len) {
-        c[u] = cp[(int64_t)(i + u) * 64];
-        v[u] = __builtin_nontemporal_load(vp + (int64_t)(i + u) * 64);
-      }
-    X xv[U][KT];
-#pragma unroll
-    for (int u = 0; u < U - 1; u++)
-      if (i + u < len)
-#pragma unroll
-        for (int j = 0; j < KT; j++) xv[u][j] = xb[(int64_t)c[u] * d.ldx + j];
-#pragma unroll
-    for (int u = 0; u < U - 1; u++)
-      if (i + u < len)
-#pragma unroll
-        for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
+    for (int j = 0; j < KT; j++) acc[j] = ar[0][j];
   }
   for (uint32_t o = 1; o < tpr; o <<= 1) {
 #pragma unroll
     for (int j = 0; j < KT; j++) {
-      Acc other = P::shfl_down(acc[j], (int)o);
+      Acc other = P::shfl_down(acc[j], (int)o, 64);
       P::add(acc[j], other);
     }
   }
@@ -194,13 +109,16 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   for (int j = 0; j < KT; j++) yr[j] = P::result(acc[j]);
 }
 
-template <class P, int KT, int U>
+template <class P, int KT, int U, int STAGE>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict__ desc,
                                                     const BatchItem *__restrict__ items,
                                                     const int32_t *__restrict__ wstart,
                                                     int32_t nwaves, int32_t ktiles) {
+  constexpr int kSlice = (STAGE > 0 && KT == 1) ? staged_bytes<P, U>() : 16;
+  __shared__ __attribute__((aligned(16))) char lds[4 * kSlice];
   const int32_t lane = threadIdx.x & 63;
-  const int32_t w = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  const int32_t wib = threadIdx.x >> 6;
+  const int32_t w = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * (blockDim.x >> 6) + wib));
   if (w >= nwaves) return;
   const int32_t it1 = wstart[w + 1];
   for (int32_t it = wstart[w]; it < it1; it++) {
@@ -214,26 +132,30 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict_
       }
       continue;
     }
-    for (int32_t kt = 0; kt < ktiles; kt++) batch_group<P, KT, U>(d, item.g, kt * KT, lane);
+    for (int32_t kt = 0; kt < ktiles; kt++)
+      batch_group<P, KT, U, STAGE>(d, item.g, kt * KT, lane, lds + wib * kSlice);
   }
 }
 
 using BatchFn = void (*)(const wk_batch *, int32_t, hipStream_t);
 
-template <class P, int KT>
+template <class P, int KT, int STAGE>
 void launch_batch(const wk_batch *B, int32_t ktiles, hipStream_t s) {
   const int32_t blocks = (B->nwaves + 3) / 4;
-  hipLaunchKernelGGL((batch_kernel<P, KT, 8>), dim3(blocks), dim3(256), 0, s, B->d_desc, B->d_items,
-                     B->d_wstart, B->nwaves, ktiles);
+  hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE>), dim3(blocks), dim3(256), 0, s, B->d_desc,
+                     B->d_items, B->d_wstart, B->nwaves, ktiles);
 }
 
 template <class P>
-BatchFn pick_batch_kt(int kt) {
+BatchFn pick_batch_kt(int kt, int stage) {
   switch (kt) {
-    case 1: return &launch_batch<P, 1>;
-    case 2: return &launch_batch<P, 2>;
-    case 4: return &launch_batch<P, 4>;
-    case 8: return &launch_batch<P, 8>;
+    case 1:
+      if (stage == 2) return &launch_batch<P, 1, 2>;
+      if (stage == 1) return &launch_batch<P, 1, 1>;
+      return &launch_batch<P, 1, 0>;
+    case 2: return &launch_batch<P, 2, 0>;
+    case 4: return &launch_batch<P, 4, 0>;
+    case 8: return &launch_batch<P, 8, 0>;
   }
   return nullptr;
 }
@@ -270,12 +192,15 @@ using namespace wk;
 
 extern "C" {
 
-int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu, wk_batch **out) {
+int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
+                    int32_t stage_lds, wk_batch **out) {
   return guarded([&] {
     if (!out || !engines || count < 1) fail(WK_ERR_ARG, "bad argument");
     *out = nullptr;
     if (waves_per_cu < 0 || waves_per_cu > 32) fail(WK_ERR_ARG, "waves_per_cu must be in [0, 32]");
+    if (stage_lds < 0 || stage_lds > 2) fail(WK_ERR_ARG, "stage_lds must be 0, 1 or 2");
     auto *B = new wk_batch();
+    B->stage = stage_lds;
     try {
       hip_check(hipGetDevice(&B->device), "hipGetDevice");
       for (int32_t i = 0; i < count; i++) {
@@ -371,7 +296,8 @@ int wk_batch_run(wk_batch *B, void *stream) {
     if (!B) fail(WK_ERR_ARG, "batch is NULL");
     if (!B->k) fail(WK_ERR_STATE, "batch has no bound vectors (wk_batch_bind)");
     if (!B->nitems) return;
-    BatchFn fn = B->dtype == WK_F64 ? pick_batch_kt<BF64>(B->kt) : pick_batch_kt<BF32C>(B->kt);
+    BatchFn fn = B->dtype == WK_F64 ? pick_batch_kt<PF64>(B->kt, B->stage)
+                                    : pick_batch_kt<PF32C>(B->kt, B->stage);
     if (!fn) fail(WK_ERR_ARG, "unsupported batch configuration");
     fn(B, B->k / B->kt, static_cast<hipStream_t>(stream));
     hip_check(hipGetLastError(), "batch launch");

@@ -24,104 +24,13 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "spmv_device.hpp"
 #include "wk_internal.hpp"
 
 namespace wk {
 namespace {
 
 constexpr int kMaxThreads = 512;
-
-template <bool NT, class T>
-__device__ __forceinline__ T ld_stream(const T *p) {
-  if constexpr (NT)
-    return __builtin_nontemporal_load(p);
-  else
-    return *p;
-}
-
-// ---------------------------------------------------------------- precision
-struct PF64 {
-  using V = double;
-  using X = double;
-  struct Acc {
-    double s;
-  };
-  // fma(v, x, -0) is the rounded product v*x bit for bit (signed zeros
-  // included), so starting at -0 and adding every slot with fma reproduces the
-  // reference's "first product, then fma" order
-  __device__ static __forceinline__ void init(Acc &a) { a.s = -0.0; }
-  __device__ static __forceinline__ void first(Acc &a, V v, X x) { a.s = v * x; }
-  __device__ static __forceinline__ void step(Acc &a, V v, X x) { a.s = __builtin_fma(v, x, a.s); }
-  __device__ static __forceinline__ void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
-  __device__ static __forceinline__ Acc shfl_down(const Acc &a, int o, int width) {
-    return Acc{__shfl_down(a.s, o, width)};
-  }
-  __device__ static __forceinline__ double result(const Acc &a) { return a.s; }
-};
-
-// FP32 values and x; error-free transformations (TwoProd by fma, Knuth's
-// TwoSum) carry the rounding errors in c (Ogita, Rump, Oishi 2005, Dot2).
-struct PF32C {
-  using V = float;
-  using X = float;
-  struct Acc {
-    float s, c;
-  };
-  __device__ static __forceinline__ void init(Acc &a) {
-    a.s = -0.0f;
-    a.c = 0.0f;
-  }
-  __device__ static __forceinline__ void first(Acc &a, V v, X x) {
-    a.s = v * x;
-    a.c = __builtin_fmaf(v, x, -a.s);
-  }
-  __device__ static __forceinline__ void two_sum(float a, float b, float &s, float &t) {
-    s = a + b;
-    float bp = s - a;
-    float ap = s - bp;
-    t = (a - ap) + (b - bp);
-  }
-  __device__ static __forceinline__ void step(Acc &a, V v, X x) {
-    float p = v * x;
-    float e = __builtin_fmaf(v, x, -p);
-    float s, t;
-    two_sum(a.s, p, s, t);
-    a.s = s;
-    a.c = a.c + (t + e);
-  }
-  __device__ static __forceinline__ void add(Acc &a, const Acc &b) {
-    float s, t;
-    two_sum(a.s, b.s, s, t);
-    a.s = s;
-    a.c = (a.c + b.c) + t;
-  }
-  __device__ static __forceinline__ Acc shfl_down(const Acc &a, int o, int width) {
-    return Acc{__shfl_down(a.s, o, width), __shfl_down(a.c, o, width)};
-  }
-  __device__ static __forceinline__ double result(const Acc &a) {
-    return (double)a.s + (double)a.c;
-  }
-};
-
-// FP32 values and x; the product of two floats is exact in FP64, so an FP64
-// fma chain gives the FP64 CSR sum of the quantised inputs.
-struct PF32D {
-  using V = float;
-  using X = float;
-  struct Acc {
-    double s;
-  };
-  __device__ static __forceinline__ void init(Acc &a) { a.s = -0.0; }
-  __device__ static __forceinline__ void first(Acc &a, V v, X x) { a.s = (double)v * (double)x; }
-  __device__ static __forceinline__ void step(Acc &a, V v, X x) {
-    a.s = __builtin_fma((double)v, (double)x, a.s);
-  }
-  __device__ static __forceinline__ void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
-  __device__ static __forceinline__ Acc shfl_down(const Acc &a, int o, int width) {
-    return Acc{__shfl_down(a.s, o, width)};
-  }
-  __device__ static __forceinline__ double result(const Acc &a) { return a.s; }
-};
 
 // ----------------------------------------------------------- block mapping
 // Workgroups are handed to the 8 XCDs round-robin by id.  With remap, the
@@ -151,80 +60,6 @@ __device__ __forceinline__ GroupMeta load_meta(const SpmvArgs &a, int32_t g) {
   m.tpr = SPLIT && m.valid ? a.tpr[g] : 1u;
   m.row0 = SPLIT ? (m.valid ? a.row_off[g] : 0u) : (uint32_t)g * (uint32_t)W;
   return m;
-}
-
-// Steps [i, end) of R groups in lockstep, U slots per group per batch.  All
-// loads of a batch are issued before the first gather waits, so a wave has
-// R*U value+index loads in flight per batch.  Values stream with the
-// nontemporal hint (NT); 4-byte index loads measured faster without it.
-template <class P, int W, int KT, int U, bool NT, int R>
-__device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
-                                                 const typename P::V *const (&vp)[R],
-                                                 const int32_t *const (&cp)[R],
-                                                 const typename P::X *xb, int64_t ldx,
-                                                 uint32_t i, uint32_t end) {
-  using V = typename P::V;
-  using X = typename P::X;
-  for (; i + U <= end; i += U) {
-    V v[R][U];
-    int32_t c[R][U];
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        c[r][u] = cp[r][(int64_t)(i + u) * W];
-        v[r][u] = ld_stream<NT>(vp[r] + (int64_t)(i + u) * W);
-      }
-    X xv[R][U][KT];
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const X *xr = xb + (int64_t)c[r][u] * ldx;
-#pragma unroll
-        for (int j = 0; j < KT; j++) xv[r][u][j] = xr[j];
-      }
-#pragma unroll
-    for (int r = 0; r < R; r++)
-#pragma unroll
-      for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int j = 0; j < KT; j++) P::step(acc[r][j], v[r][u], xv[r][u][j]);
-  }
-  return i;
-}
-
-// The last, partial batch of one group: steps [i, end) with end - i < U, all
-// loads issued before the first wait (the guards are wave-uniform for W = 64).
-template <class P, int W, int KT, int U, bool NT>
-__device__ __forceinline__ void stream_tail(typename P::Acc (&acc)[KT], const typename P::V *vp,
-                                            const int32_t *cp, const typename P::X *xb,
-                                            int64_t ldx, uint32_t i, uint32_t end) {
-  using V = typename P::V;
-  using X = typename P::X;
-  if (i >= end) return;
-  int32_t c[U];
-  V v[U];
-#pragma unroll
-  for (int u = 0; u < U - 1; u++)
-    if (i + u < end) {
-      c[u] = cp[(int64_t)(i + u) * W];
-      v[u] = ld_stream<NT>(vp + (int64_t)(i + u) * W);
-    }
-  X xv[U][KT];
-#pragma unroll
-  for (int u = 0; u < U - 1; u++)
-    if (i + u < end) {
-      const X *xr = xb + (int64_t)c[u] * ldx;
-#pragma unroll
-      for (int j = 0; j < KT; j++) xv[u][j] = xr[j];
-    }
-#pragma unroll
-  for (int u = 0; u < U - 1; u++)
-    if (i + u < end) {
-#pragma unroll
-      for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
-    }
 }
 
 // grid.x = nwork + fill blocks.  Work block -> (group block, k-tile), k-tile
@@ -343,13 +178,6 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_kernel(SpmvArgs a, int32_t n
 // loads are limited by the texture-address rate, wide ones reach the HBM
 // ceiling.  Only the wave touches its slice, so no barrier is needed: LDS
 // operations of one wave complete in order.
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-template <class P, int U>
-constexpr int staged_bytes() {
-  return U * 64 * (int)(sizeof(typename P::V) + sizeof(int32_t));
-}
-
 template <class P, bool SPLIT, int OUT, int U, bool PIPE>
 __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, int32_t nwork,
                                                                   int32_t remap) {
@@ -357,9 +185,6 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, in
   using X = typename P::X;
   using Acc = typename P::Acc;
   constexpr int W = 64;
-  constexpr int NV = U * 64 * (int)sizeof(V) / 1024;  // 16-byte loads per lane, values
-  constexpr int NC = U * 64 * 4 / 1024;               // 16-byte loads per lane, indices
-  static_assert(NV >= 1 && NC >= 1 && U % 4 == 0, "a batch covers whole 1 KB chunks");
   extern __shared__ __attribute__((aligned(16))) char lds[];
   const int32_t lane = threadIdx.x & 63;
   const int32_t wpb = blockDim.x >> 6;
@@ -374,82 +199,13 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, in
   const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(blockIdx.x, nwork, remap) * wpb + wib);
   if (wave >= a.ngroups) return;
   const GroupMeta m = load_meta<W, SPLIT>(a, wave);
-  V *lv = reinterpret_cast<V *>(lds + wib * staged_bytes<P, U>());  // [U][64]
-  int32_t *lc = reinterpret_cast<int32_t *>(lds + wib * staged_bytes<P, U>() + U * 64 * sizeof(V));
   const V *vbase = static_cast<const V *>(a.vals) + m.base;
   const int32_t *cbase = a.cols + m.base;
   const X *__restrict__ xb = static_cast<const X *>(a.x);
   const int64_t ldx = a.ldx;
 
-  Acc acc;
-  P::init(acc);  // every slot by fma onto -0 (see PF64::init)
-  uint32_t i = 0;
-  if constexpr (!PIPE) {
-    for (; i + U <= m.len; i += U) {
-      const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)i * W);
-      const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)i * W);
-      u32x4 tc[NC], tv[NV];
-#pragma unroll
-      for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
-#pragma unroll
-      for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
-#pragma unroll
-      for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
-#pragma unroll
-      for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
-      X xv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
-#pragma unroll
-      for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
-    }
-  } else if (i + U <= m.len) {
-    // Depth-2 pipeline: once a batch sits in LDS its registers are free, so
-    // the next batch's 16-byte loads are issued right after this batch's
-    // gathers and stay in flight through the FMAs.  The loop body is
-    // straight-line and both edges into its header carry the same loads
-    // into the same registers.
-    u32x4 tc[NC], tv[NV];
-    auto issue = [&](uint32_t at) {
-      const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)at * W);
-      const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)at * W);
-#pragma unroll
-      for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
-#pragma unroll
-      for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
-    };
-    auto stage = [&]() {
-#pragma unroll
-      for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
-#pragma unroll
-      for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
-    };
-    issue(i);
-    for (; i + 2 * U <= m.len; i += U) {
-      stage();
-      X xv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
-      issue(i + U);
-      V v[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) v[u] = lv[u * 64 + lane];
-#pragma unroll
-      for (int u = 0; u < U; u++) P::step(acc, v[u], xv[u]);
-    }
-    stage();
-    X xv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
-#pragma unroll
-    for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
-    i += U;
-  }
-  {
-    Acc ar[1] = {acc};
-    stream_tail<P, W, 1, U, true>(ar, vbase + lane, cbase + lane, xb, ldx, i, m.len);
-    acc = ar[0];
-  }
+  Acc acc = staged_group<P, U, PIPE>(vbase, cbase, xb, ldx, m.len,
+                                      lds + wib * staged_bytes<P, U>(), lane);
   if constexpr (SPLIT) {
     for (uint32_t o = 1; o < m.tpr; o <<= 1) {
       Acc other = P::shfl_down(acc, (int)o, W);

@@ -1,0 +1,283 @@
+// spmv_device.hpp -- device code shared by the SpMV kernels (kernels.hip)
+// and the batched persistent kernel (batch.hip): the per-precision
+// arithmetic, the direct batch loops and the LDS-staged group loop.
+//
+// Summation order per lane is the reference's (warpkernel.hpp:68-71): the
+// rounded first product, then one fma per further slot, trailing pads
+// included.  Starting from -0 and adding every slot with fma gives exactly
+// that (see PF64::init).  Compiled with -ffp-contract=off: every fusion is an
+// explicit fma.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace wk {
+namespace {
+
+template <bool NT, class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+// ---------------------------------------------------------------- precision
+struct PF64 {
+  using V = double;
+  using X = double;
+  struct Acc {
+    double s;
+  };
+  // fma(v, x, -0) is the rounded product v*x bit for bit (signed zeros
+  // included), so starting at -0 and adding every slot with fma reproduces the
+  // reference's "first product, then fma" order
+  __device__ static __forceinline__ void init(Acc &a) { a.s = -0.0; }
+  __device__ static __forceinline__ void first(Acc &a, V v, X x) { a.s = v * x; }
+  __device__ static __forceinline__ void step(Acc &a, V v, X x) { a.s = __builtin_fma(v, x, a.s); }
+  __device__ static __forceinline__ void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
+  __device__ static __forceinline__ Acc shfl_down(const Acc &a, int o, int width) {
+    return Acc{__shfl_down(a.s, o, width)};
+  }
+  __device__ static __forceinline__ double result(const Acc &a) { return a.s; }
+};
+
+// FP32 values and x; error-free transformations (TwoProd by fma, Knuth's
+// TwoSum) carry the rounding errors in c (Ogita, Rump, Oishi 2005, Dot2).
+struct PF32C {
+  using V = float;
+  using X = float;
+  struct Acc {
+    float s, c;
+  };
+  __device__ static __forceinline__ void init(Acc &a) {
+    a.s = -0.0f;
+    a.c = 0.0f;
+  }
+  __device__ static __forceinline__ void first(Acc &a, V v, X x) {
+    a.s = v * x;
+    a.c = __builtin_fmaf(v, x, -a.s);
+  }
+  __device__ static __forceinline__ void two_sum(float a, float b, float &s, float &t) {
+    s = a + b;
+    float bp = s - a;
+    float ap = s - bp;
+    t = (a - ap) + (b - bp);
+  }
+  __device__ static __forceinline__ void step(Acc &a, V v, X x) {
+    float p = v * x;
+    float e = __builtin_fmaf(v, x, -p);
+    float s, t;
+    two_sum(a.s, p, s, t);
+    a.s = s;
+    a.c = a.c + (t + e);
+  }
+  __device__ static __forceinline__ void add(Acc &a, const Acc &b) {
+    float s, t;
+    two_sum(a.s, b.s, s, t);
+    a.s = s;
+    a.c = (a.c + b.c) + t;
+  }
+  __device__ static __forceinline__ Acc shfl_down(const Acc &a, int o, int width) {
+    return Acc{__shfl_down(a.s, o, width), __shfl_down(a.c, o, width)};
+  }
+  __device__ static __forceinline__ double result(const Acc &a) {
+    return (double)a.s + (double)a.c;
+  }
+};
+
+// FP32 values and x; the product of two floats is exact in FP64, so an FP64
+// fma chain gives the FP64 CSR sum of the quantised inputs.
+struct PF32D {
+  using V = float;
+  using X = float;
+  struct Acc {
+    double s;
+  };
+  __device__ static __forceinline__ void init(Acc &a) { a.s = -0.0; }
+  __device__ static __forceinline__ void first(Acc &a, V v, X x) { a.s = (double)v * (double)x; }
+  __device__ static __forceinline__ void step(Acc &a, V v, X x) {
+    a.s = __builtin_fma((double)v, (double)x, a.s);
+  }
+  __device__ static __forceinline__ void add(Acc &a, const Acc &b) { a.s = a.s + b.s; }
+  __device__ static __forceinline__ Acc shfl_down(const Acc &a, int o, int width) {
+    return Acc{__shfl_down(a.s, o, width)};
+  }
+  __device__ static __forceinline__ double result(const Acc &a) { return a.s; }
+};
+
+// Steps [i, end) of R groups in lockstep, U slots per group per batch.  All
+// loads of a batch are issued before the first gather waits, so a wave has
+// R*U value+index loads in flight per batch.  Values stream with the
+// nontemporal hint (NT); 4-byte index loads measured faster without it.
+template <class P, int W, int KT, int U, bool NT, int R>
+__device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
+                                                 const typename P::V *const (&vp)[R],
+                                                 const int32_t *const (&cp)[R],
+                                                 const typename P::X *xb, int64_t ldx,
+                                                 uint32_t i, uint32_t end) {
+  using V = typename P::V;
+  using X = typename P::X;
+  for (; i + U <= end; i += U) {
+    V v[R][U];
+    int32_t c[R][U];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        c[r][u] = cp[r][(int64_t)(i + u) * W];
+        v[r][u] = ld_stream<NT>(vp[r] + (int64_t)(i + u) * W);
+      }
+    X xv[R][U][KT];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const X *xr = xb + (int64_t)c[r][u] * ldx;
+#pragma unroll
+        for (int j = 0; j < KT; j++) xv[r][u][j] = xr[j];
+      }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < KT; j++) P::step(acc[r][j], v[r][u], xv[r][u][j]);
+  }
+  return i;
+}
+
+// The last, partial batch of one group: steps [i, end) with end - i < U, all
+// loads issued before the first wait (the guards are wave-uniform for W = 64).
+template <class P, int W, int KT, int U, bool NT>
+__device__ __forceinline__ void stream_tail(typename P::Acc (&acc)[KT], const typename P::V *vp,
+                                            const int32_t *cp, const typename P::X *xb,
+                                            int64_t ldx, uint32_t i, uint32_t end) {
+  using V = typename P::V;
+  using X = typename P::X;
+  if (i >= end) return;
+  int32_t c[U];
+  V v[U];
+#pragma unroll
+  for (int u = 0; u < U - 1; u++)
+    if (i + u < end) {
+      c[u] = cp[(int64_t)(i + u) * W];
+      v[u] = ld_stream<NT>(vp + (int64_t)(i + u) * W);
+    }
+  X xv[U][KT];
+#pragma unroll
+  for (int u = 0; u < U - 1; u++)
+    if (i + u < end) {
+      const X *xr = xb + (int64_t)c[u] * ldx;
+#pragma unroll
+      for (int j = 0; j < KT; j++) xv[u][j] = xr[j];
+    }
+#pragma unroll
+  for (int u = 0; u < U - 1; u++)
+    if (i + u < end) {
+#pragma unroll
+      for (int j = 0; j < KT; j++) P::step(acc[j], v[u], xv[u][j]);
+    }
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <class P, int U>
+constexpr int staged_bytes() {
+  return U * 64 * (int)(sizeof(typename P::V) + sizeof(int32_t));
+}
+
+// All steps of one group through the wave's LDS slice (U steps per batch,
+// the last partial batch by direct loads); PIPE issues the next batch's loads
+// before the current batch's FMAs.  Returns the lane's accumulator.
+template <class P, int U, bool PIPE>
+__device__ __forceinline__ typename P::Acc staged_group(const typename P::V *vbase,
+                                                        const int32_t *cbase,
+                                                        const typename P::X *__restrict__ xb,
+                                                        int64_t ldx, uint32_t len, char *slice,
+                                                        int32_t lane) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  constexpr int W = 64;
+  constexpr int NV = U * 64 * (int)sizeof(V) / 1024;  // 16-byte loads per lane, values
+  constexpr int NC = U * 64 * 4 / 1024;               // 16-byte loads per lane, indices
+  static_assert(NV >= 1 && NC >= 1 && U % 4 == 0, "a batch covers whole 1 KB chunks");
+  V *lv = reinterpret_cast<V *>(slice);                                 // [U][64]
+  int32_t *lc = reinterpret_cast<int32_t *>(slice + U * 64 * sizeof(V));  // [U][64]
+  Acc acc;
+  P::init(acc);  // every slot by fma onto -0 (see PF64::init)
+  uint32_t i = 0;
+  if constexpr (!PIPE) {
+    for (; i + U <= len; i += U) {
+      const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)i * W);
+      const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)i * W);
+      u32x4 tc[NC], tv[NV];
+#pragma unroll
+      for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
+#pragma unroll
+      for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+#pragma unroll
+      for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
+#pragma unroll
+      for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
+      X xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
+#pragma unroll
+      for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+    }
+  } else if (i + U <= len) {
+    // Depth-2 pipeline: once a batch sits in LDS its registers are free, so
+    // the next batch's 16-byte loads are issued right after this batch's
+    // gathers and stay in flight through the FMAs.  The loop body is
+    // straight-line and both edges into its header carry the same loads
+    // into the same registers.
+    u32x4 tc[NC], tv[NV];
+    auto issue = [&](uint32_t at) {
+      const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)at * W);
+      const u32x4 *gc = reinterpret_cast<const u32x4 *>(cbase + (int64_t)at * W);
+#pragma unroll
+      for (int q = 0; q < NC; q++) tc[q] = gc[q * 64 + lane];
+#pragma unroll
+      for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+    };
+    auto stage = [&]() {
+#pragma unroll
+      for (int q = 0; q < NC; q++) reinterpret_cast<u32x4 *>(lc)[q * 64 + lane] = tc[q];
+#pragma unroll
+      for (int q = 0; q < NV; q++) reinterpret_cast<u32x4 *>(lv)[q * 64 + lane] = tv[q];
+    };
+    issue(i);
+    for (; i + 2 * U <= len; i += U) {
+      stage();
+      X xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
+      issue(i + U);
+      V v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = lv[u * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < U; u++) P::step(acc, v[u], xv[u]);
+    }
+    stage();
+    X xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = xb[(int64_t)lc[u * 64 + lane] * ldx];
+#pragma unroll
+    for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+    i += U;
+  }
+  {
+    Acc ar[1] = {acc};
+    stream_tail<P, W, 1, U, true>(ar, vbase + lane, cbase + lane, xb, ldx, i, len);
+    acc = ar[0];
+  }
+  return acc;
+}
+
+}  // namespace
+}  // namespace wk
