@@ -225,7 +225,40 @@ def extra_batch(torch, dev, steps, warmup, tune=True):
     return out
 
 
-EXTRAS = {"multi_rhs": extra_multi_rhs, "fwd_adjoint": extra_fwd_adjoint, "batch": extra_batch}
+def extra_orderings(torch, dev, steps, warmup, tune=True):
+    """P1 Kuhn tets (96^3 cubes, 5% of the elements dropped) in lexicographic
+    order and under a random node permutation (x-gather stress), and the 2D Q1
+    Laplacian (128^2, cache-resident), WPK1 and WPK2, run and run_x."""
+    from warpkernel_amd import engine, fe, structure, structure2
+    out = {}
+    for name in ("tet_poisson_3d_96", "tet_poisson_3d_96_perm", "q1_laplace_2d_128"):
+        m = fe.CONFIGS[name]()
+        nb = spmv_bytes(m.nnz, m.nrows, m.ncols)
+        x = fe.uniform_x(m.ncols, seed=2300)
+        dx = dev(x)
+        y = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+        row = {"rows": m.nrows, "nnz": m.nnz, "empty_rows": int((m.row_lengths() == 0).sum())}
+        for variant, th in (("wpk1", None), ("wpk2", 5)):
+            s = (structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind) if th is None else
+                 structure2().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, th))
+            eng = engine(s, m.vals, m.colind, kr=True)
+            dxr = dev(s.reorder_x(x))
+            if tune:
+                eng.tune(dx, y, reps=5)
+                eng.tune(dxr, y, op=1, reps=5)
+            _, ms = time_launches(torch, lambda: eng.launch(dx, y), steps, warmup)
+            _, msx = time_launches(torch, lambda: eng.launch_x(dxr, y), steps, warmup)
+            key = variant if th is None else f"{variant}_t{th}"
+            row[f"{key}_run_GBs"] = round(nb / (ms * 1e-3) / 1e9, 1)
+            row[f"{key}_run_x_GBs"] = round(nb / (msx * 1e-3) / 1e9, 1)
+            row[f"{key}_run_us"] = round(ms * 1e3, 2)
+            del eng
+        out[name] = row
+    return out
+
+
+EXTRAS = {"multi_rhs": extra_multi_rhs, "fwd_adjoint": extra_fwd_adjoint, "batch": extra_batch,
+          "orderings": extra_orderings}
 
 
 def main():
@@ -317,15 +350,21 @@ def main():
     if not args.headline_only:
         dxr = dev(s.reorder_x(x))
         dyr = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
-        eng.tune(dxr, dyr, op=1, reps=10)
+        if not args.no_tune:
+            eng.tune(dxr, dyr, op=1, reps=10)
         _, msx = time_launches(torch, lambda: eng.launch_x(dxr, dyr), args.steps, args.warmup)
         ex = {"run_x_GBs": round(nbytes / (msx * 1e-3) / 1e9, 2), "run_x_ms": round(msx, 4)}
         del eng
         s2 = structure2(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, args.threshold)
-        e2 = engine(s2, m.vals, m.colind)
-        e2.tune(dx, dy, reps=10)
+        e2 = engine(s2, m.vals, m.colind, kr=True)
+        dxr2 = dev(s2.reorder_x(x))
+        if not args.no_tune:
+            e2.tune(dx, dy, reps=10)
+            e2.tune(dxr2, dyr, op=1, reps=10)
         _, ms2 = time_launches(torch, lambda: e2.launch(dx, dy), args.steps, args.warmup)
+        _, ms2x = time_launches(torch, lambda: e2.launch_x(dxr2, dyr), args.steps, args.warmup)
         ex[f"wpk2_t{args.threshold}_GBs"] = round(nbytes / (ms2 * 1e-3) / 1e9, 2)
+        ex[f"wpk2_t{args.threshold}_run_x_GBs"] = round(nbytes / (ms2x * 1e-3) / 1e9, 2)
         del e2
         print("[bench] extras", file=sys.stderr, flush=True)
         for fn in EXTRAS.values():

@@ -236,7 +236,8 @@ typedef struct wk_batch wk_batch;
 
 /* waves_per_cu: persistent waves per compute unit (0: 8).  stage_lds: as in
  * wk_launch, for batches with one right-hand side (0 direct loads, 1 staged,
- * 2 staged with the next batch in flight). */
+ * 2 staged with the next batch in flight); applied to the matrices whose mean
+ * group length is at least 24, the others use direct loads. */
 int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
                     int32_t stage_lds, wk_batch **out);
 

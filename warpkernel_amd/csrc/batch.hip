@@ -29,6 +29,7 @@ struct BatchDesc {
   double *y;
   int64_t ldx, ldy;
   int32_t nznrows, nrows, ngroups, split;
+  int32_t staged;  // stream this matrix's groups through LDS (long rows)
 };
 
 struct BatchItem {
@@ -79,9 +80,14 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   const int32_t *cbase = d.cols + base;
   const X *xb = static_cast<const X *>(d.x) + j0;
   Acc acc[KT];
+  bool done = false;
   if constexpr (STAGE > 0 && KT == 1) {
-    acc[0] = staged_group<P, U, STAGE == 2>(vbase, cbase, xb, d.ldx, len, slice, lane);
-  } else {
+    if (d.staged) {
+      acc[0] = staged_group<P, U, STAGE == 2>(vbase, cbase, xb, d.ldx, len, slice, lane);
+      done = true;
+    }
+  }
+  if (!done) {
     (void)slice;
 #pragma unroll
     for (int j = 0; j < KT; j++) P::init(acc[j]);
@@ -255,6 +261,8 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       d.nrows = D.nrows;
       d.ngroups = D.ngroups;
       d.split = D.variant == 2;
+      // staging pays off for long rows only (it adds LDS traffic per batch)
+      d.staged = D.ngroups && (double)D.allocate_nz / ((double)D.ngroups * 64.0) >= 24.0;
       if ((D.nrows && !y_devs[i]) || (E->ncols && !x_devs[i])) fail(WK_ERR_ARG, "NULL vector");
       for (int32_t g = 0; g < D.ngroups; g++) {
         items.push_back({(int32_t)i, g});
