@@ -257,8 +257,47 @@ def extra_orderings(torch, dev, steps, warmup, tune=True):
     return out
 
 
+def extra_device_scan(torch, dev, steps, warmup, tune=True):
+    """Scan + process of the 64^3 hex elasticity matrix: on the host (scan,
+    then engine() with its process and upload, the reference's path) against
+    wk_engine_create_device from a CSR already in HBM; both give the same
+    engine, compared here array by array."""
+    from warpkernel_amd import _capi as capi, engine, fe, structure, structure2
+    del steps, warmup, tune
+    m = fe.CONFIGS["hex_elastic_3d_64"]()
+    rp, ci, va = dev(m.rowptr), dev(m.colind), dev(m.vals)
+    out = {"rows": m.nrows, "nnz": m.nnz}
+    for variant in (1, 2):
+        th = 27 if variant == 2 else 0
+        host_s, dev_s = [], []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if variant == 1:
+                s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+            else:
+                s = structure2().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, th)
+            eh = engine(s, m.vals, m.colind, kr=True)
+            torch.cuda.synchronize()
+            host_s.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            ed = engine.from_device(rp, ci, va, variant=variant, threshold=th, kr=True)
+            torch.cuda.synchronize()
+            dev_s.append(time.perf_counter() - t0)
+        same = all(np.array_equal(eh.download(w), ed.download(w)) for w in
+                   (capi.WK_ROW_MAP, capi.WK_REORDER_ROWS, capi.WK_WARP_OFFSETS,
+                    capi.WK_ENGINE_VALUES, capi.WK_ENGINE_COLS, capi.WK_ENGINE_COLS_KR))
+        key = "wpk1" if variant == 1 else f"wpk2_t{th}"
+        out[key] = {"host_scan_process_upload_s": round(min(host_s), 4),
+                    "device_scan_process_s": round(min(dev_s), 4),
+                    "speedup": round(min(host_s) / min(dev_s), 1), "identical_engines": bool(same)}
+        eh.close()
+        ed.close()
+    return out
+
+
 EXTRAS = {"multi_rhs": extra_multi_rhs, "fwd_adjoint": extra_fwd_adjoint, "batch": extra_batch,
-          "orderings": extra_orderings}
+          "orderings": extra_orderings, "device_scan": extra_device_scan}
 
 
 def main():

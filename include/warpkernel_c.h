@@ -148,6 +148,32 @@ int wk_engine_update_values(wk_engine *engine, const void *vals_dev, void *strea
 int wk_engine_set_colinds(wk_engine *engine, const int32_t *pcols, int32_t where, void *stream);
 int wk_engine_get_colinds(const wk_engine *engine, int32_t *pcols_host);
 
+/* structure::scan + process (warpkernel.hpp:309-365, 532-548) or
+ * structure2::scan(.., threshold) + process (hpp:578-736, 749-769) with the
+ * CSR already on the device: the sort by row length, the grouping and the
+ * padded arrays are computed on the GPU (reorder.cu:138-166 moves only the
+ * scatter there).  The result equals wk_scan1 / wk_scan2 at group width 64
+ * followed by wk_engine_create, bit for bit.  variant 1 or 2 (threshold and
+ * max_tpr as wk_scan2; ignored for 1); flags: WK_ENGINE_KR only.  The CSR is
+ * checked on the device (WK_ERR_CSR); rows longer than 8191 entries give
+ * WK_ERR_LIMIT.  Synchronises `stream`. */
+int wk_engine_create_device(int32_t variant, int32_t threshold, int32_t max_tpr, int32_t nrows,
+                            int32_t ncols, int64_t nnz, const int32_t *rowptr_dev,
+                            const int32_t *colind_dev, const double *vals_dev,
+                            int32_t values_dtype, uint32_t flags, void *stream, wk_engine **out);
+
+/* engine arrays (wk_engine_download), besides WK_ROW_MAP .. WK_ROW_OFFSET_WARP
+ * (WK_REORDER_ROWS gives the CSR-to-slot map; WK_ROW_MAP_INV exists for
+ * engines built by wk_engine_create_device) */
+#define WK_ENGINE_VALUES 16   /* double or float [allocate_nz]  device_values   hpp:782 */
+#define WK_ENGINE_COLS 17     /* int32 [allocate_nz]            device_colinds  hpp:783 */
+#define WK_ENGINE_COLS_KR 18  /* int32 [allocate_nz]            Kr columns (WK_ENGINE_KR) */
+
+/* Copies one of the engine's device arrays of the forward layout to the host.
+ * host_dst NULL: *count receives the array's length (0 for an array the engine
+ * lacks); else *count must equal it. */
+int wk_engine_download(const wk_engine *engine, int32_t which, void *host_dst, int64_t *count);
+
 /* Launch options; wk_launch_default fills the built-in choice. */
 typedef struct wk_launch {
   int32_t waves_per_block; /* 1, 2, 4 or 8 (the drivers' warps/block sweep) */

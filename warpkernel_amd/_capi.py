@@ -18,6 +18,7 @@ WK_ACC_TWOSUM, WK_ACC_F64 = 0, 1
 WK_ROW_MAP, WK_ROW_MAP_INV, WK_REORDER_ROWS, WK_WARP_OFFSETS = 0, 1, 2, 3
 WK_MAX_NZ_ROWS, WK_THREADS_PER_ROW, WK_ROW_OFFSET_WARP = 4, 5, 6
 WK_ENGINE_KR, WK_ENGINE_ADJOINT = 1, 2
+WK_ENGINE_VALUES, WK_ENGINE_COLS, WK_ENGINE_COLS_KR = 16, 17, 18
 
 
 class WKError(RuntimeError):
@@ -70,6 +71,9 @@ SIGNATURES = {
     "wk_engine_update_values": (C.c_int, [P, P, P]),
     "wk_engine_set_colinds": (C.c_int, [P, P, I32, P]),
     "wk_engine_get_colinds": (C.c_int, [P, P]),
+    "wk_engine_create_device": (C.c_int, [I32, I32, I32, I32, I32, I64, P, P, P, I32, C.c_uint32, P,
+                                          C.POINTER(P)]),
+    "wk_engine_download": (C.c_int, [P, I32, P, C.POINTER(I64)]),
     "wk_launch_default": (None, [C.POINTER(Launch)]),
     "wk_run": (C.c_int, [P, C.POINTER(Launch), P, P, I32, I64, I64, P]),
     "wk_run_x": (C.c_int, [P, C.POINTER(Launch), P, P, I32, I64, I64, P]),

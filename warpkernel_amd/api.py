@@ -189,6 +189,54 @@ class engine:
         self.nrows, self.ncols, self.nz = k.nrows, k.ncols, k.nz
         self.has_kr, self.has_adjoint = kr, adjoint
 
+    @classmethod
+    def from_device(cls, rowptr, colind, vals, ncols: int | None = None, variant: int = 1,
+                    threshold: int = 0, max_tpr: int = 32, dtype: str = "f64", kr: bool = False,
+                    stream=None) -> "engine":
+        """Scan and process on the GPU: rowptr / colind (int32) and vals (float64)
+        are device tensors of one CSR.  The layout is structure's (variant 1) or
+        structure2's with `threshold` (variant 2) at group width 64, the same bit
+        for bit as a host scan followed by engine(); it stays on the device
+        (engine.kernel exposes its scalars and downloads its arrays)."""
+        import torch
+        for t, dt in ((rowptr, torch.int32), (colind, torch.int32), (vals, torch.float64)):
+            if not (t.is_cuda and t.dtype == dt and t.is_contiguous()):
+                raise ValueError("rowptr/colind must be contiguous int32 and vals float64 device tensors")
+        self = cls.__new__(cls)
+        self._h = None
+        self.dtype = {"f64": capi.WK_F64, "f32": capi.WK_F32}[dtype]
+        nrows = rowptr.numel() - 1
+        nnz = colind.numel()
+        ncols = nrows if ncols is None else int(ncols)
+        if vals.numel() != nnz:
+            raise ValueError("vals and colind differ in length")
+        h = C.c_void_p()
+        check(lib().wk_engine_create_device(int(variant), int(threshold), int(max_tpr), nrows, ncols,
+                                            nnz, _dev_ptr(rowptr), _dev_ptr(colind), _dev_ptr(vals),
+                                            self.dtype, capi.WK_ENGINE_KR if kr else 0,
+                                            _stream(stream), C.byref(h)))
+        self._h = h
+        self.kernel = _EngineLayout(self)
+        self.nrows, self.ncols, self.nz = nrows, ncols, nnz
+        self.has_kr, self.has_adjoint = kr, False
+        return self
+
+    def download(self, which: int) -> np.ndarray:
+        """One device array of the forward layout (WK_ROW_MAP .. WK_ROW_OFFSET_WARP,
+        WK_ENGINE_VALUES / COLS / COLS_KR); WK_REORDER_ROWS is the CSR-to-slot map."""
+        dt = {capi.WK_ROW_MAP: np.int32, capi.WK_ROW_MAP_INV: np.int32,
+              capi.WK_REORDER_ROWS: np.uint32, capi.WK_WARP_OFFSETS: np.int64,
+              capi.WK_MAX_NZ_ROWS: np.uint32, capi.WK_THREADS_PER_ROW: np.uint32,
+              capi.WK_ROW_OFFSET_WARP: np.uint32,
+              capi.WK_ENGINE_VALUES: np.float64 if self.dtype == capi.WK_F64 else np.float32,
+              capi.WK_ENGINE_COLS: np.int32, capi.WK_ENGINE_COLS_KR: np.int32}[which]
+        n = C.c_int64()
+        check(lib().wk_engine_download(self._h, which, None, C.byref(n)))
+        out = np.empty(n.value, dt)
+        if n.value:
+            check(lib().wk_engine_download(self._h, which, _host_ptr(out), C.byref(n)))
+        return out
+
     # -- launch configuration ---------------------------------------------
     def get_launch(self, op: int = capi.WK_OP_RUN, k: int = 1) -> capi.Launch:
         """The configuration used for (op, k) when no cfg is passed."""
@@ -338,6 +386,33 @@ class engine:
             self.close()
         except Exception:
             pass
+
+
+class _EngineLayout:
+    """The layout of an engine scanned on the device: structure's scalars, and
+    its arrays downloaded on first use."""
+
+    _ARRAYS = {"row_map": capi.WK_ROW_MAP, "row_map_inv": capi.WK_ROW_MAP_INV,
+               "reorder_rows": capi.WK_REORDER_ROWS, "warp_offsets": capi.WK_WARP_OFFSETS,
+               "max_nz_rows": capi.WK_MAX_NZ_ROWS, "threads_per_row": capi.WK_THREADS_PER_ROW,
+               "row_offset_warp": capi.WK_ROW_OFFSET_WARP}
+
+    def __init__(self, eng: "engine"):
+        self._eng = eng
+        info = capi.LayoutInfo()
+        check(lib().wk_engine_get_info(eng._h, C.byref(info)))
+        self.info = info
+        self.variant, self.group_width = info.variant, info.group_width
+        self.nrows, self.ncols, self.nznrows = info.nrows, info.ncols, info.nznrows
+        self.nz, self.allocate_nz, self.nwarps = info.nz, info.allocate_nz, info.nwarps
+        self.min_nz, self.max_nz = info.min_nz, info.max_nz
+
+    def __getattr__(self, name):
+        if name in _EngineLayout._ARRAYS:
+            a = self._eng.download(_EngineLayout._ARRAYS[name])
+            setattr(self, name, a)
+            return a
+        raise AttributeError(name)
 
 
 def _verify(orig, comp, row_map) -> bool:

@@ -166,23 +166,6 @@ BatchFn pick_batch_kt(int kt, int stage) {
   return nullptr;
 }
 
-template <class F>
-int guarded(F &&f) {
-  try {
-    f();
-    return WK_OK;
-  } catch (const Error &e) {
-    set_last_error(e.msg);
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    set_last_error("host allocation failed");
-    return WK_ERR_NOMEM;
-  } catch (const std::exception &e) {
-    set_last_error(e.what());
-    return WK_ERR_ARG;
-  }
-}
-
 void free_batch(wk_batch *B) {
   if (!B) return;
   (void)hipFree(B->d_desc);

@@ -24,39 +24,6 @@ void hip_check(hipError_t e, const char *what) {
 
 namespace {
 
-template <class F>
-int guarded(F &&f) {
-  try {
-    f();
-    return WK_OK;
-  } catch (const Error &e) {
-    set_last_error(e.msg);
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    set_last_error("host allocation failed");
-    return WK_ERR_NOMEM;
-  } catch (const std::exception &e) {
-    set_last_error(e.what());
-    return WK_ERR_ARG;
-  }
-}
-
-template <class T>
-T *dev_alloc(wk_engine *E, int64_t count) {
-  if (count <= 0) return nullptr;
-  void *p = nullptr;
-  hip_check(hipMalloc(&p, sizeof(T) * (size_t)count), "hipMalloc");
-  E->owned.push_back(p);
-  return static_cast<T *>(p);
-}
-
-template <class T>
-T *dev_upload(wk_engine *E, const T *src, int64_t count, hipStream_t s) {
-  T *d = dev_alloc<T>(E, count);
-  if (d) hip_check(hipMemcpyAsync(d, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, s), "upload");
-  return d;
-}
-
 void free_engine(wk_engine *E) {
   if (!E) return;
   // teardown is best effort: errors here have nowhere to go

@@ -123,6 +123,58 @@ void scan1(wk_layout &L, int32_t W, int32_t nrows, int32_t ncols, int64_t nnz,
   });
 }
 
+void split_groups(const std::vector<uint32_t> &aw0, int32_t nzr, int32_t W, int32_t threshold,
+                  int32_t max_tpr, std::vector<uint32_t> &max_nz_rows,
+                  std::vector<uint32_t> &rows_nominal, std::vector<uint32_t> &rows_actual) {
+  // Forward pass over the W-row groups of sorted rows.  A group whose longest
+  // row exceeds T gets i lanes per row (smallest power of two with
+  // A_w <= cap*i, cap = T or ceil(A_w/max_tpr)), each lane a contiguous chunk
+  // of nz_per_thread = ceil(A_w/i) entries.  A full group becomes i sub-groups
+  // of W/i rows; the last, partial group is split the same way unless its rows
+  // times i fall short of W, in which case it stays one group with W/i nominal
+  // rows (hpp:660-662).  Sub-groups of a split count W/i rows each for
+  // row_offset_warp (hpp:666), even in a partial last group.
+  const uint32_t T = (uint32_t)threshold;
+  max_nz_rows.clear();
+  rows_nominal.clear();
+  rows_actual.clear();
+  const int32_t G0 = (int32_t)aw0.size();
+  for (int32_t g = 0; g < G0; g++) {
+    const uint32_t aw = aw0[g];
+    const uint32_t rows = (uint32_t)std::min<int64_t>(W, nzr - (int64_t)g * W);
+    // smallest i in {2, 4, .., max_tpr} with aw <= cap*i (uint32 arithmetic as
+    // in the reference); none qualifies only if cap*i wraps, and then the
+    // group is left whole, as the reference's loop leaves it
+    uint32_t i = 0;
+    if (aw > T) {
+      const uint32_t cap = aw > T * (uint32_t)max_tpr ? (aw + max_tpr - 1) / max_tpr : T;
+      for (uint32_t c = 2; c <= (uint32_t)max_tpr; c *= 2)
+        if (aw <= cap * c) {
+          i = c;
+          break;
+        }
+    }
+    if (i) {
+      const uint32_t nzpt = (aw + i - 1) / i;
+      if (rows * i < (uint32_t)W) {
+        max_nz_rows.push_back(nzpt);
+        rows_nominal.push_back(W / i);
+        rows_actual.push_back(rows);
+      } else {
+        for (uint32_t j = 0; j < i; j++) {
+          max_nz_rows.push_back(nzpt);
+          rows_nominal.push_back(W / i);
+          rows_actual.push_back(W / i);
+        }
+      }
+    } else {
+      max_nz_rows.push_back(aw);
+      rows_nominal.push_back(W);
+      rows_actual.push_back(rows);
+    }
+  }
+}
+
 void scan2(wk_layout &L, int32_t W, int32_t threshold, int32_t max_tpr, int32_t nrows,
            int32_t ncols, int64_t nnz, const int32_t *rowptr) {
   check_width(W);
@@ -145,51 +197,11 @@ void scan2(wk_layout &L, int32_t W, int32_t threshold, int32_t max_tpr, int32_t 
   L.min_nz = nzr > 0 ? std::min<uint32_t>(slen[nzr - 1], (uint32_t)nrows) : (uint32_t)nrows;
   L.max_nz = nzr > 0 ? slen[0] : 0;
 
-  // Forward pass over the W-row groups of sorted rows.  A group whose longest
-  // row exceeds T gets i lanes per row (smallest power of two with
-  // A_w <= cap*i, cap = T or ceil(A_w/max_tpr)), each lane a contiguous chunk
-  // of nz_per_thread = ceil(A_w/i) entries.  A full group becomes i sub-groups
-  // of W/i rows; the last, partial group is split the same way unless its rows
-  // times i fall short of W, in which case it stays one group with W/i nominal
-  // rows (hpp:660-662).  Sub-groups of a split count W/i rows each for
-  // row_offset_warp (hpp:666), even in a partial last group.
-  const uint32_t T = (uint32_t)threshold;
   const int32_t G0 = (nzr + W - 1) / W;
+  std::vector<uint32_t> aw0((size_t)G0);
+  for (int32_t g = 0; g < G0; g++) aw0[g] = slen[(size_t)g * W];
   std::vector<uint32_t> rows_nominal, rows_actual;
-  for (int32_t g = 0; g < G0; g++) {
-    const uint32_t aw = slen[(size_t)g * W];
-    const uint32_t rows = (uint32_t)std::min<int64_t>(W, nzr - (int64_t)g * W);
-    // smallest i in {2, 4, .., max_tpr} with aw <= cap*i (uint32 arithmetic as
-    // in the reference); none qualifies only if cap*i wraps, and then the
-    // group is left whole, as the reference's loop leaves it
-    uint32_t i = 0;
-    if (aw > T) {
-      const uint32_t cap = aw > T * (uint32_t)max_tpr ? (aw + max_tpr - 1) / max_tpr : T;
-      for (uint32_t c = 2; c <= (uint32_t)max_tpr; c *= 2)
-        if (aw <= cap * c) {
-          i = c;
-          break;
-        }
-    }
-    if (i) {
-      const uint32_t nzpt = (aw + i - 1) / i;
-      if (rows * i < (uint32_t)W) {
-        L.max_nz_rows.push_back(nzpt);
-        rows_nominal.push_back(W / i);
-        rows_actual.push_back(rows);
-      } else {
-        for (uint32_t j = 0; j < i; j++) {
-          L.max_nz_rows.push_back(nzpt);
-          rows_nominal.push_back(W / i);
-          rows_actual.push_back(W / i);
-        }
-      }
-    } else {
-      L.max_nz_rows.push_back(aw);
-      rows_nominal.push_back(W);
-      rows_actual.push_back(rows);
-    }
-  }
+  split_groups(aw0, nzr, W, threshold, max_tpr, L.max_nz_rows, rows_nominal, rows_actual);
   L.nwarps = (int32_t)L.max_nz_rows.size();
   const int32_t G = L.nwarps;
   finish_offsets(L);

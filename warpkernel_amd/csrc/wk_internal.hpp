@@ -71,6 +71,7 @@ struct DevLayout {
   int32_t *cols = nullptr;        // [allocate_nz]
   int32_t *cols_kr = nullptr;     // [allocate_nz] Kr columns (optional)
   int32_t *row_map = nullptr;     // [nrows]
+  int32_t *row_map_inv = nullptr; // [nrows] (engines scanned on the device)
   int64_t *warp_offsets = nullptr;   // [ngroups+1]
   uint32_t *max_nz_rows = nullptr;   // [ngroups]
   uint32_t *tpr = nullptr;           // [ngroups] (variant 2)
@@ -140,5 +141,48 @@ int64_t dvals_tile_size();
 // the tuned one if any, else a choice from the layout.
 wk_launch engine_launch(wk_engine *E, int32_t op, int32_t k);
 wk_launch heuristic_launch(const DevLayout &D, int32_t k);
+
+// WPK2's group split (warpkernel.hpp:633-691) over the longest row of each
+// W-row group of sorted rows (aw0); nzr = non-empty rows.  Outputs per
+// (sub-)group: A_w, rows per group for threads_per_row, rows it advances
+// row_offset_warp by.
+void split_groups(const std::vector<uint32_t> &aw0, int32_t nzr, int32_t W, int32_t threshold,
+                  int32_t max_tpr, std::vector<uint32_t> &max_nz_rows,
+                  std::vector<uint32_t> &rows_nominal, std::vector<uint32_t> &rows_actual);
+
+// Runs f and turns what it throws into a status code and wk_last_error().
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return WK_OK;
+  } catch (const Error &e) {
+    set_last_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return WK_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return WK_ERR_ARG;
+  }
+}
+
+// Device allocations owned (and freed) by an engine.
+template <class T>
+T *dev_alloc(wk_engine *E, int64_t count) {
+  if (count <= 0) return nullptr;
+  void *p = nullptr;
+  hip_check(hipMalloc(&p, sizeof(T) * (size_t)count), "hipMalloc");
+  E->owned.push_back(p);
+  return static_cast<T *>(p);
+}
+
+template <class T>
+T *dev_upload(wk_engine *E, const T *src, int64_t count, hipStream_t s) {
+  T *d = dev_alloc<T>(E, count);
+  if (d) hip_check(hipMemcpyAsync(d, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, s), "upload");
+  return d;
+}
 
 }  // namespace wk
