@@ -393,6 +393,10 @@ def main():
             eng.tune(dxr, dyr, op=1, reps=10)
         _, msx = time_launches(torch, lambda: eng.launch_x(dxr, dyr), args.steps, args.warmup)
         ex = {"run_x_GBs": round(nbytes / (msx * 1e-3) / 1e9, 2), "run_x_ms": round(msx, 4)}
+        # run_assembly: row sums of the values, bytes = nnz*8 read + nrows*8 written
+        _, msa = time_launches(torch, lambda: eng.launch_assembly(dyr), args.steps, args.warmup)
+        ex["run_assembly_GBs"] = round((m.nnz * 8 + m.nrows * 8) / (msa * 1e-3) / 1e9, 2)
+        ex["run_assembly_ms"] = round(msa, 4)
         del eng
         s2 = structure2(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, args.threshold)
         e2 = engine(s2, m.vals, m.colind, kr=True)

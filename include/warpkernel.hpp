@@ -298,6 +298,11 @@ class engine {
   float run_x(int nblocks, int blocksize, const ValueType *dx, double *dy) {
     return timed<cache>(1, nblocks, blocksize, dx, dy);
   }
+  // engine::run_assembly<cache> (hpp:898-929): row sums of the padded values.
+  template <bool cache>
+  float run_assembly(int nblocks, int blocksize, double *dy) {
+    return timed<cache>(2, nblocks, blocksize, nullptr, dy);
+  }
 
   // Asynchronous forms with k right-hand sides (x [ncols][ldx], y [nrows][ldy]).
   void launch(const ValueType *dx, double *dy, int k = 1, int64_t ldx = 1, int64_t ldy = 1,
@@ -352,7 +357,7 @@ class engine {
     (void)nblocks;
     if (blocksize <= 0 || blocksize % 64) throw error(WK_ERR_ARG, "blocksize must be a multiple of 64");
     wk_launch c;
-    check(wk_engine_get_launch(h_, which == 0 ? WK_OP_RUN : WK_OP_RUN_X, 1, &c));
+    check(wk_engine_get_launch(h_, which == 1 ? WK_OP_RUN_X : WK_OP_RUN, 1, &c));
     c.waves_per_block = blocksize / 64 > 8 ? 8 : blocksize / 64;
     if (!cache) {
       // plain loads for everything: the direct kernel, one row per lane

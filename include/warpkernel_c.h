@@ -214,6 +214,15 @@ int wk_residual(wk_engine *engine, const wk_launch *cfg, const double *b_dev, in
                 const double *x_dev, double *r_dev, int32_t k, int64_t ldx, int64_t ldr,
                 void *stream);
 
+/* engine::run_assembly<cache> (warpkernel.hpp:898-929, kernel hpp:78-100):
+ * y[row_map[p]] = the padded values of sorted row p summed in slot order
+ * (first value, then one add per slot, pads included) in the engine's value
+ * type; y is FP64 [nrows], rows without entries get 0.  WPK2 layouts (no
+ * reference kernel) sum each lane's chunk and combine the lanes of a row in
+ * warpKernel2's pairwise order.  cfg (NULL = the engine's run configuration)
+ * supplies waves_per_block and xcd_remap. */
+int wk_run_assembly(wk_engine *engine, const wk_launch *cfg, double *y_dev, void *stream);
+
 /* dx = A^T dy through the transposed WPK layout (WK_ENGINE_ADJOINT); no
  * atomics.  dy: [nrows][lddy], dx: [ncols][lddx].  No reference counterpart. */
 int wk_run_adjoint(wk_engine *engine, const wk_launch *cfg, const void *dy_dev, double *dx_dev,
@@ -292,7 +301,7 @@ void wk_graph_destroy(wk_graph *graph);
 /* Blocking timing helper with the reference's contract (engine::run returns
  * the seconds of one launch, warpkernel.hpp:829-843): launches once, waits,
  * returns the elapsed seconds measured with HIP events in *seconds.
- * which: 0 run, 1 run_x. */
+ * which: 0 run, 1 run_x, 2 run_assembly (x unused, k = 1). */
 int wk_time_run(wk_engine *engine, const wk_launch *cfg, int32_t which, const void *x_dev,
                 double *y_dev, int32_t k, int64_t ldx, int64_t ldy, void *stream,
                 float *seconds);

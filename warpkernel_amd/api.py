@@ -279,7 +279,7 @@ class engine:
 
     # -- reference-shaped, blocking, returns seconds --------------------
     def _timed(self, which, blocksize, dx, dy, cfg, stream):
-        op = capi.WK_OP_RUN if which == 0 else capi.WK_OP_RUN_X
+        op = capi.WK_OP_RUN_X if which == 1 else capi.WK_OP_RUN
         c = self._cfg(op, 1, cfg or {})
         if blocksize:
             c.waves_per_block = max(1, min(8, int(blocksize) // 64))
@@ -299,6 +299,12 @@ class engine:
         """engine::run_x<cache>: the Kr form, y'[i] for x' = reorder_x(x)."""
         del nblocks
         return self._timed(1, blocksize, dx, dy, cfg, stream)
+
+    def run_assembly(self, nblocks, blocksize, dy, cfg=None, stream=None) -> float:
+        """engine::run_assembly<cache> (hpp:898-929): row sums of the padded
+        values into dy (original row order); seconds of one launch."""
+        del nblocks
+        return self._timed(2, blocksize, None, dy, cfg, stream)
 
     # -- asynchronous forms ---------------------------------------------
     @staticmethod
@@ -321,6 +327,11 @@ class engine:
         _, ldy = self._k_ld(y)
         c = self._cfg(capi.WK_OP_RUN_X, k, cfg)
         check(lib().wk_run_x(self._h, self._ref(c), _dev_ptr(x), _dev_ptr(y), k, ldx, ldy, _stream(stream)))
+
+    def launch_assembly(self, y, cfg=None, stream=None):
+        """Row sums of the padded values into y (FP64, original row order)."""
+        c = self._cfg(capi.WK_OP_RUN, 1, cfg)
+        check(lib().wk_run_assembly(self._h, self._ref(c), _dev_ptr(y), _stream(stream)))
 
     def residual(self, b, x, r, cfg=None, stream=None):
         """r = b - A x in FP64."""

@@ -376,6 +376,15 @@ int wk_dvals(wk_engine *E, const double *dy_dev, int64_t lddy, const double *x_d
   });
 }
 
+int wk_run_assembly(wk_engine *E, const wk_launch *cfg, double *y_dev, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (!y_dev && E->nrows) fail(WK_ERR_ARG, "NULL vector");
+    launch_rowsum(E->fwd, E->dtype, resolve(E, WK_OP_RUN, 1, cfg), y_dev,
+                  static_cast<hipStream_t>(stream));
+  });
+}
+
 int wk_time_run(wk_engine *E, const wk_launch *cfg, int32_t which, const void *x_dev,
                 double *y_dev, int32_t k, int64_t ldx, int64_t ldy, void *stream, float *seconds) {
   return guarded([&] {
@@ -387,8 +396,10 @@ int wk_time_run(wk_engine *E, const wk_launch *cfg, int32_t which, const void *x
     hip_check(hipEventCreate(&e1), "event");
     int rc;
     hip_check(hipEventRecord(e0, s), "event");
-    rc = which == 0 ? wk_run(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
-                    : wk_run_x(E, cfg, x_dev, y_dev, k, ldx, ldy, stream);
+    if (which == 2 && k != 1) fail(WK_ERR_ARG, "run_assembly has one output column");
+    rc = which == 0   ? wk_run(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
+         : which == 1 ? wk_run_x(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
+                      : wk_run_assembly(E, cfg, y_dev, stream);
     const hipError_t r1 = hipEventRecord(e1, s);
     const hipError_t r2 = r1 == hipSuccess ? hipEventSynchronize(e1) : r1;
     float ms = 0;

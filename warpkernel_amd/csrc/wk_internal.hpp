@@ -129,6 +129,10 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
 void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
                            hipStream_t stream);
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
+// Row sums of the padded values (run_assembly); cfg supplies waves_per_block
+// and xcd_remap.
+void launch_rowsum(const DevLayout &D, int32_t dtype, const wk_launch &cfg, double *y,
+                   hipStream_t stream);
 void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *tile_row,
                   int32_t nrows, int64_t nnz, const double *dy, int64_t lddy, const double *x,
                   int64_t ldx, int32_t k, double *dvals, hipStream_t stream);
