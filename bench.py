@@ -253,6 +253,14 @@ def extra_orderings(torch, dev, steps, warmup, tune=True):
             row[f"{key}_run_x_GBs"] = round(nb / (msx * 1e-3) / 1e9, 1)
             row[f"{key}_run_us"] = round(ms * 1e3, 2)
             del eng
+            if th is None:
+                # Kr row sort: each row's x gathers in ascending permuted order
+                er = engine(s, m.vals, m.colind, kr=True, rowsort=True)
+                if tune:
+                    er.tune(dxr, y, op=1, reps=5)
+                _, msr = time_launches(torch, lambda: er.launch_x(dxr, y), steps, warmup)
+                row[f"{key}_rowsort_run_x_GBs"] = round(nb / (msr * 1e-3) / 1e9, 1)
+                del er
         out[name] = row
     return out
 

@@ -124,6 +124,7 @@ class structure {
   std::vector<uint> reorder_rows;  // new_values[reorder_rows[i]] = A[i]
   std::vector<int64_t> warp_offsets;
   std::vector<uint> max_nz_rows;
+  std::vector<int> reorder_A_rows;  // after reorder_columns_rowsort (hpp:279)
   uint nrows = 0, nznrows = 0, nz = 0, allocate_nz = 0, nwarps = 0;
   uint ncols = 0;
 
@@ -155,6 +156,28 @@ class structure {
     std::vector<IndexType> out(allocate_nz);
     check(wk_layout_reorder_columns_coalesced(h_, colinds.data(), out.data()));
     colinds.swap(out);
+  }
+
+  // reorder_columns_rowsort (hpp:414-444): colinds (CSR order, nz entries)
+  // becomes the permuted columns with each row ascending; reorder_A_rows[n]
+  // is the CSR index of the entry now at n.  Equal columns keep CSR order.
+  template <typename IndexType>
+  void reorder_columns_rowsort(std::vector<IndexType> &colinds, const std::vector<IndexType> &rows) {
+    static_assert(std::is_same<IndexType, int>::value, "IndexType must be int");
+    std::vector<IndexType> out(colinds.size());
+    check(wk_layout_reorder_columns_rowsort(h_, rows.data(), colinds.data(), out.data()));
+    colinds.swap(out);
+    reorder_A_rows = array<int>(WK_REORDER_A_ROWS);
+  }
+
+  // reorder_values_coalesced_rowsort (hpp:505-515): padded values,
+  // dvalues[reorder_rows[i]] = values[reorder_A_rows[i]].
+  template <typename ValueType>
+  void reorder_values_coalesced_rowsort(const std::vector<ValueType> &values,
+                                        device_vector<ValueType> &dvalues) const {
+    std::vector<double> v(values.begin(), values.end()), pv(allocate_nz);
+    check(wk_layout_reorder_values_rowsort(h_, v.data(), pv.data()));
+    dvalues = std::vector<ValueType>(pv.begin(), pv.end());
   }
 
   // reorder_x (hpp:517-530): device_x[row_map_inv[r]] = x[r].

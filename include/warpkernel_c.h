@@ -59,10 +59,13 @@ extern "C" {
 #define WK_MAX_NZ_ROWS 4     /* uint32 [nwarps]                                   hpp:281 */
 #define WK_THREADS_PER_ROW 5 /* uint32 [nwarps]    structure2 only                hpp:562 */
 #define WK_ROW_OFFSET_WARP 6 /* uint32 [nwarps+1]  structure2 only                hpp:561 */
+#define WK_REORDER_A_ROWS 7  /* int32 [nz] after wk_layout_reorder_columns_rowsort  hpp:279 */
 
 /* wk_engine_create flags */
 #define WK_ENGINE_KR 1u       /* also build the Kr (run_x) column indices */
 #define WK_ENGINE_ADJOINT 2u  /* also build the transposed layout (run_adjoint, dvals) */
+#define WK_ENGINE_ROWSORT 4u  /* each row's entries in permuted-column order (square
+                                 matrices; see wk_layout_reorder_columns_rowsort) */
 
 const char *wk_last_error(void);
 int wk_abi_version(void);
@@ -116,6 +119,23 @@ int wk_layout_reorder_columns_coalesced(const wk_layout *layout, const int32_t *
 /* structure::reorder_x (warpkernel.hpp:517-530): xout[row_map_inv[r]] = x[r]. */
 int wk_layout_reorder_x(const wk_layout *layout, const double *x_host, double *xout_host);
 
+/* structure::reorder_columns_rowsort (warpkernel.hpp:414-444), for the Kr form
+ * of a square matrix: within each row the entries are ordered by permuted
+ * column row_map_inv[col] ascending (equal columns keep CSR order, where the
+ * reference's std::sort leaves them unspecified).  cols_out_host[n] (nz
+ * entries, CSR order) receives the permuted columns; the layout keeps
+ * reorder_A_rows[n] = the CSR index of the entry now at n (WK_REORDER_A_ROWS).
+ * An engine made from vals[reorder_A_rows] and these columns runs run_x in
+ * that order (optimize.cu:645-704); WK_ENGINE_ROWSORT builds the same engine
+ * from the original CSR. */
+int wk_layout_reorder_columns_rowsort(wk_layout *layout, const int32_t *rowptr_host,
+                                      const int32_t *colind_host, int32_t *cols_out_host);
+
+/* structure::reorder_values_coalesced_rowsort (warpkernel.hpp:505-515): zero-
+ * filled padded values, pvals[reorder_rows[i]] = vals[reorder_A_rows[i]]. */
+int wk_layout_reorder_values_rowsort(const wk_layout *layout, const double *vals_host,
+                                     double *pvals_host);
+
 void wk_layout_destroy(wk_layout *layout);
 
 /* ------------------------------------------------------------------ engine */
@@ -144,7 +164,8 @@ int wk_engine_update_values(wk_engine *engine, const void *vals_dev, void *strea
 
 /* The reference's drivers overwrite the public engine::device_colinds with the
  * Kr columns and restore them (wpk1.cu:160-197).  pcols has allocate_nz
- * entries in padded order; where = 0 host, 1 device. */
+ * entries in padded order; where = 0 host, 1 device.  Every entry must lie in
+ * [0, ncols) (checked before the copy; WK_ERR_ARG otherwise). */
 int wk_engine_set_colinds(wk_engine *engine, const int32_t *pcols, int32_t where, void *stream);
 int wk_engine_get_colinds(const wk_engine *engine, int32_t *pcols_host);
 

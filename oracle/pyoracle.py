@@ -47,6 +47,8 @@ def lib():
         L.wko_process.argtypes = [C.POINTER(_Layout), P, P, P, P]
         L.wko_reorder_columns_coalesced.argtypes = [C.POINTER(_Layout), P, P]
         L.wko_reorder_x.argtypes = [C.POINTER(_Layout), P, P]
+        L.wko_reorder_columns_rowsort.argtypes = [C.POINTER(_Layout), P, P, P, P]
+        L.wko_reorder_values_rowsort.argtypes = [C.POINTER(_Layout), P, P, P]
         L.wko_csr_spmv.argtypes = [C.c_int32, P, P, P, P, P, C.c_int32, C.c_int64, C.c_int64, C.c_int32]
         L.wko_csr_spmv_omp.argtypes = [C.c_int32, P, P, P, P, P, C.c_int32]
         L.wko_wpk_spmv.argtypes = [C.POINTER(_Layout), P, P, P, P, C.c_int32, C.c_int64, C.c_int64, C.c_int32]
@@ -152,6 +154,25 @@ def reorder_x(L: Layout, x: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=np.float64)
     out = np.empty_like(x)
     lib().wko_reorder_x(L._handle, _p(x), _p(out))
+    return out
+
+
+def reorder_columns_rowsort(L: Layout, rowptr, colind):
+    """Returns (cols_out, reorder_A_rows): CSR-order permuted columns, each row
+    sorted ascending, and the source CSR index of every entry."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    colind = np.ascontiguousarray(colind, dtype=np.int32)
+    out = np.empty(colind.size, np.int32)
+    src = np.empty(colind.size, np.int32)
+    lib().wko_reorder_columns_rowsort(L._handle, _p(rowptr), _p(colind), _p(out), _p(src))
+    return out, src
+
+
+def reorder_values_rowsort(L: Layout, vals, reorder_A_rows) -> np.ndarray:
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    src = np.ascontiguousarray(reorder_A_rows, dtype=np.int32)
+    out = np.empty(L.allocate_nz, np.float64)
+    lib().wko_reorder_values_rowsort(L._handle, _p(vals), _p(src), _p(out))
     return out
 
 

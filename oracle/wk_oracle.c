@@ -284,6 +284,49 @@ void wko_reorder_x(const wko_layout *L, const double *x, double *xout) {
   for (int32_t r = 0; r < L->nrows; r++) xout[L->row_map_inv[r]] = x[r];
 }
 
+typedef struct {
+  int32_t key; /* permuted column */
+  int32_t at;  /* position within the row */
+} wko_colkey;
+
+static int colkey_cmp(const void *a, const void *b) {
+  const wko_colkey *p = (const wko_colkey *)a, *q = (const wko_colkey *)b;
+  if (p->key != q->key) return p->key < q->key ? -1 : 1;
+  return p->at < q->at ? -1 : (p->at > q->at);
+}
+
+/* warpkernel.hpp:414-444: pairs (n - rows[r], row_map_inv[colinds[n]]) sorted by
+ * the second member (checkPairInOrder, hpp:228-230), ties by position. */
+void wko_reorder_columns_rowsort(const wko_layout *L, const int32_t *rowptr,
+                                 const int32_t *colind, int32_t *cols_out,
+                                 int32_t *reorder_A_rows) {
+  int32_t maxlen = 0;
+  for (int32_t r = 0; r < L->nrows; r++)
+    if (rowptr[r + 1] - rowptr[r] > maxlen) maxlen = rowptr[r + 1] - rowptr[r];
+  wko_colkey *tmp = (wko_colkey *)malloc(sizeof(wko_colkey) * (size_t)(maxlen ? maxlen : 1));
+  if (!tmp) return;
+  for (int32_t r = 0; r < L->nrows; r++) {
+    const int32_t b = rowptr[r], n = rowptr[r + 1] - rowptr[r];
+    for (int32_t i = 0; i < n; i++) {
+      tmp[i].key = L->row_map_inv[colind[b + i]];
+      tmp[i].at = i;
+    }
+    qsort(tmp, (size_t)n, sizeof(wko_colkey), colkey_cmp);
+    for (int32_t i = 0; i < n; i++) {
+      cols_out[b + i] = tmp[i].key;
+      reorder_A_rows[b + i] = b + tmp[i].at;
+    }
+  }
+  free(tmp);
+}
+
+/* warpkernel.hpp:505-515 */
+void wko_reorder_values_rowsort(const wko_layout *L, const double *vals,
+                                const int32_t *reorder_A_rows, double *pvals) {
+  memset(pvals, 0, sizeof(double) * (size_t)L->allocate_nz);
+  for (int64_t i = 0; i < L->nz; i++) pvals[L->reorder_rows[i]] = vals[reorder_A_rows[i]];
+}
+
 static void csr_rows(int32_t r0, int32_t r1, const int32_t *rowptr, const int32_t *colind,
                      const double *vals, const double *x, double *y, int32_t k, int64_t ldx,
                      int64_t ldy, int32_t fused) {

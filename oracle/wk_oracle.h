@@ -59,6 +59,21 @@ void wko_reorder_columns_coalesced(const wko_layout *L, const int32_t *colind, i
 /* reorder_x (warpkernel.hpp:517-530): xout[row_map_inv[r]] = x[r]. */
 void wko_reorder_x(const wko_layout *L, const double *x, double *xout);
 
+/* reorder_columns_rowsort (warpkernel.hpp:414-444): within each row, the
+ * entries ordered by permuted column row_map_inv[col] ascending;
+ * cols_out[n] = that permuted column, reorder_A_rows[n] = the CSR index the
+ * entry came from.  The reference's std::sort leaves the order of equal
+ * columns (duplicates within a row) unspecified; here they keep CSR order.
+ * Pinned by hand-worked cases (tests/golden/kat_wpk.json) and by the
+ * permutation / sortedness properties. */
+void wko_reorder_columns_rowsort(const wko_layout *L, const int32_t *rowptr,
+                                 const int32_t *colind, int32_t *cols_out,
+                                 int32_t *reorder_A_rows);
+/* reorder_values_coalesced_rowsort (warpkernel.hpp:505-515): zero-filled
+ * padded values, pvals[reorder_rows[i]] = vals[reorder_A_rows[i]]. */
+void wko_reorder_values_rowsort(const wko_layout *L, const double *vals,
+                                const int32_t *reorder_A_rows, double *pvals);
+
 /* FP64 CSR SpMV in original row order, k right-hand sides (x is [ncols][ldx],
  * y is [nrows][ldy], row-major).  fused = 1: first product rounded, then fma
  * (the order of warpKernel, warpkernel.hpp:68-71, with contraction as nvcc does

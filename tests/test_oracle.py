@@ -266,3 +266,40 @@ def test_transpose_and_dvals():
         fp = 0.5 * np.sum((O.csr_spmv(rp, ci, vp, x) - b) ** 2)
         fm = 0.5 * np.sum((O.csr_spmv(rp, ci, vm, x) - b) ** 2)
         assert abs((fp - fm) / (2 * h) - g[j]) <= 1e-7 * max(1.0, abs(g[j]))
+
+
+@pytest.mark.parametrize("idx", range(2))
+def test_oracle_rowsort_golden(golden, idx):
+    case = golden["rowsort_cases"][idx]
+    rp = np.array(case["rowptr"], np.int32)
+    cols = np.array(case["colind"], np.int32)
+    L = O.scan1(rp, case["W"])
+    out, src = O.reorder_columns_rowsort(L, rp, cols)
+    assert out.tolist() == case["expect"]["cols_out"]
+    assert src.tolist() == case["expect"]["reorder_A_rows"]
+    vals = np.arange(1.0, cols.size + 1.0)
+    pv = O.reorder_values_rowsort(L, vals, src)
+    assert np.array_equal(pv[L.reorder_rows], vals[src])
+    assert np.count_nonzero(pv) == cols.size
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_oracle_rowsort_properties(seed):
+    """Each row of the output is its own entries (a permutation that stays in the
+    row), their permuted columns ascending, ties in CSR order."""
+    rng = np.random.default_rng(900 + seed)
+    n = 700
+    rp, cols, vals = random_csr(rng, n, n, seed)
+    cols[rp[1]:rp[1] + 3] = cols[rp[1]] if rp[2] - rp[1] >= 3 else cols[rp[1]:rp[1] + 3]
+    L = O.scan1(rp, 64)
+    out, src = O.reorder_columns_rowsort(L, rp, cols)
+    assert np.array_equal(np.sort(src), np.arange(cols.size))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert np.array_equal(rows[src], rows)
+    assert np.array_equal(out, L.row_map_inv[cols[src]])
+    for r in range(n):
+        k = out[rp[r]:rp[r + 1]]
+        s = src[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(k) >= 0)
+        ties = np.diff(k) == 0
+        assert np.all(np.diff(s)[ties] > 0)

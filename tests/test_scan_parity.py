@@ -109,3 +109,46 @@ def test_empty_and_all_empty_matrices():
         assert (s.nznrows, s.nwarps, s.allocate_nz) == (0, 0, 0)
         s2 = structure2().scan(0, nrows, None, rp, None, 3)
         assert list(s2.row_offset_warp) == [nrows]
+
+
+def test_rowsort_golden_through_product(golden):
+    # the row order (row_map_inv) does not depend on the group width, so the
+    # W = 4 cases hold at the product's W = 64
+    for case in golden["rowsort_cases"]:
+        rp = np.array(case["rowptr"], np.int32)
+        cols = np.array(case["colind"], np.int32)
+        s = structure(64).scan(int(rp[-1]), rp.size - 1, None, rp, cols)
+        out = s.reorder_columns_rowsort(cols)
+        assert out.tolist() == case["expect"]["cols_out"], case["name"]
+        assert s.reorder_A_rows.tolist() == case["expect"]["reorder_A_rows"], case["name"]
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("variant", [1, 2])
+def test_rowsort_matches_oracle(seed, variant):
+    rng = np.random.default_rng(950 + seed)
+    n = 1500
+    rp, cols, vals = random_csr(rng, n, n, seed)
+    if variant == 1:
+        s = structure(64).scan(int(rp[-1]), n, vals, rp, cols)
+        L = O.scan1(rp, 64)
+    else:
+        s = structure2(64, 32).scan(int(rp[-1]), n, vals, rp, cols, 3)
+        L = O.scan2(rp, 3, 64, 32)
+    out = s.reorder_columns_rowsort(cols, rp)
+    want, src = O.reorder_columns_rowsort(L, rp, cols)
+    assert np.array_equal(out, want)
+    assert np.array_equal(s.reorder_A_rows, src)
+    assert np.array_equal(s.reorder_values_coalesced_rowsort(vals), O.reorder_values_rowsort(L, vals, src))
+
+
+def test_rowsort_needs_square_and_a_prior_sort():
+    rp = np.array([0, 1, 2], np.int32)
+    cols = np.array([0, 2], np.int32)
+    s = structure(64).scan(2, 2, None, rp, cols, ncols=3)
+    with pytest.raises(WKError):
+        s.reorder_columns_rowsort(cols)
+    s = structure(64).scan(2, 2, None, rp, np.array([1, 0], np.int32))
+    with pytest.raises(WKError) as e:
+        s.reorder_values_coalesced_rowsort(np.ones(2))
+    assert "reorder_columns_rowsort" in str(e.value)

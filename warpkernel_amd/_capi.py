@@ -16,8 +16,8 @@ WK_ERR_ARG, WK_ERR_CSR, WK_ERR_LIMIT, WK_ERR_HIP, WK_ERR_STATE, WK_ERR_NOMEM = 1
 WK_F64, WK_F32 = 0, 1
 WK_ACC_TWOSUM, WK_ACC_F64 = 0, 1
 WK_ROW_MAP, WK_ROW_MAP_INV, WK_REORDER_ROWS, WK_WARP_OFFSETS = 0, 1, 2, 3
-WK_MAX_NZ_ROWS, WK_THREADS_PER_ROW, WK_ROW_OFFSET_WARP = 4, 5, 6
-WK_ENGINE_KR, WK_ENGINE_ADJOINT = 1, 2
+WK_MAX_NZ_ROWS, WK_THREADS_PER_ROW, WK_ROW_OFFSET_WARP, WK_REORDER_A_ROWS = 4, 5, 6, 7
+WK_ENGINE_KR, WK_ENGINE_ADJOINT, WK_ENGINE_ROWSORT = 1, 2, 4
 WK_ENGINE_VALUES, WK_ENGINE_COLS, WK_ENGINE_COLS_KR = 16, 17, 18
 
 
@@ -64,6 +64,8 @@ SIGNATURES = {
     "wk_layout_process_host": (C.c_int, [P, P, P, P, P]),
     "wk_layout_reorder_columns_coalesced": (C.c_int, [P, P, P]),
     "wk_layout_reorder_x": (C.c_int, [P, P, P]),
+    "wk_layout_reorder_columns_rowsort": (C.c_int, [P, P, P, P]),
+    "wk_layout_reorder_values_rowsort": (C.c_int, [P, P, P]),
     "wk_layout_destroy": (None, [P]),
     "wk_engine_create": (C.c_int, [P, P, P, P, I32, U32, P, PP]),
     "wk_engine_destroy": (None, [P]),

@@ -123,6 +123,27 @@ class structure:
         check(lib().wk_layout_reorder_columns_coalesced(self._h, _host_ptr(colinds), _host_ptr(out)))
         return out
 
+    def reorder_columns_rowsort(self, colinds, rows=None):
+        """structure::reorder_columns_rowsort(colinds, rows) (hpp:414-444): CSR-order
+        permuted columns, each row ascending; sets self.reorder_A_rows (the CSR
+        index of the entry now at each position).  rows defaults to the scanned
+        rowptr."""
+        colinds = np.ascontiguousarray(colinds, dtype=np.int32)
+        rows = self._rowptr if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+        out = np.empty(colinds.size, np.int32)
+        check(lib().wk_layout_reorder_columns_rowsort(self._h, _host_ptr(rows), _host_ptr(colinds),
+                                                      _host_ptr(out)))
+        self.reorder_A_rows = self._array(capi.WK_REORDER_A_ROWS, np.int32)
+        return out
+
+    def reorder_values_coalesced_rowsort(self, values):
+        """reorder_values_coalesced_rowsort (hpp:505-515): padded values,
+        new[reorder_rows[i]] = values[reorder_A_rows[i]]."""
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        out = np.empty(self.allocate_nz, np.float64)
+        check(lib().wk_layout_reorder_values_rowsort(self._h, _host_ptr(values), _host_ptr(out)))
+        return out
+
     def reorder_x(self, x):
         """structure::reorder_x (hpp:517-530): x'[row_map_inv[r]] = x[r]."""
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -170,18 +191,21 @@ class engine:
 
     engine(k, A, colinds) runs process (hpp:794-809).  dtype 'f64' or 'f32'
     (FP32 values and x, FP64 y); kr=True also builds the Kr columns for run_x;
-    adjoint=True builds the transposed layout for run_adjoint and dvals.
+    adjoint=True builds the transposed layout for run_adjoint and dvals;
+    rowsort=True orders each row's entries by permuted column
+    (reorder_columns_rowsort, hpp:414-444) for both run and run_x.
     """
 
     def __init__(self, k: structure, A, colinds, dtype: str = "f64", kr: bool = False,
-                 adjoint: bool = False, stream=None):
+                 adjoint: bool = False, rowsort: bool = False, stream=None):
         if k._h is None:
             raise ValueError("structure has not been scanned")
         self.kernel = k
         self.dtype = {"f64": capi.WK_F64, "f32": capi.WK_F32}[dtype]
         A = np.ascontiguousarray(A, dtype=np.float64)
         colinds = np.ascontiguousarray(colinds, dtype=np.int32)
-        flags = (capi.WK_ENGINE_KR if kr else 0) | (capi.WK_ENGINE_ADJOINT if adjoint else 0)
+        flags = ((capi.WK_ENGINE_KR if kr else 0) | (capi.WK_ENGINE_ADJOINT if adjoint else 0)
+                 | (capi.WK_ENGINE_ROWSORT if rowsort else 0))
         h = C.c_void_p()
         check(lib().wk_engine_create(k._h, _host_ptr(k._rowptr), _host_ptr(colinds), _host_ptr(A),
                                      self.dtype, flags, _stream(stream), C.byref(h)))

@@ -166,6 +166,8 @@ int wk_layout_array(const wk_layout *L, int32_t which, const void **data, int64_
       case WK_ROW_OFFSET_WARP:
         if (L->variant != 2) fail(WK_ERR_STATE, "row_offset_warp exists for structure2 only");
         *data = L->row_offset_warp.data(); *count = (int64_t)L->row_offset_warp.size(); break;
+      case WK_REORDER_A_ROWS:
+        *data = L->reorder_A_rows.data(); *count = (int64_t)L->reorder_A_rows.size(); break;
       default: fail(WK_ERR_ARG, "unknown layout array");
     }
   });
@@ -196,6 +198,29 @@ int wk_layout_reorder_x(const wk_layout *L, const double *x_host, double *xout_h
   });
 }
 
+int wk_layout_reorder_columns_rowsort(wk_layout *L, const int32_t *rowptr_host,
+                                      const int32_t *colind_host, int32_t *cols_out_host) {
+  return guarded([&] {
+    if (!L || !rowptr_host || !cols_out_host || (L->nz && !colind_host)) fail(WK_ERR_ARG, "NULL argument");
+    if (L->nrows != L->ncols) fail(WK_ERR_ARG, "the row sort needs a square matrix");
+    validate_csr(L->nrows, L->ncols, L->nz, rowptr_host, colind_host);
+    std::vector<int32_t> src((size_t)L->nz);
+    reorder_columns_rowsort(*L, rowptr_host, colind_host, cols_out_host, src.data());
+    L->reorder_A_rows.swap(src);
+  });
+}
+
+int wk_layout_reorder_values_rowsort(const wk_layout *L, const double *vals_host,
+                                     double *pvals_host) {
+  return guarded([&] {
+    if (!L || !pvals_host || (L->nz && !vals_host)) fail(WK_ERR_ARG, "NULL argument");
+    if ((int64_t)L->reorder_A_rows.size() != L->nz)
+      fail(WK_ERR_STATE, "call wk_layout_reorder_columns_rowsort first");
+    std::memset(pvals_host, 0, sizeof(double) * (size_t)L->allocate_nz);
+    for (int64_t i = 0; i < L->nz; i++) pvals_host[L->reorder_rows[i]] = vals_host[L->reorder_A_rows[i]];
+  });
+}
+
 void wk_layout_destroy(wk_layout *L) { delete L; }
 
 int wk_engine_create(const wk_layout *L, const int32_t *rowptr_host, const int32_t *colind_host,
@@ -207,9 +232,14 @@ int wk_engine_create(const wk_layout *L, const int32_t *rowptr_host, const int32
     if (!L || !rowptr_host || !colind_host || !vals_host) fail(WK_ERR_ARG, "NULL argument");
     if (values_dtype != WK_F64 && values_dtype != WK_F32) fail(WK_ERR_ARG, "unknown values dtype");
     if (values_dtype == WK_F32 && L->W != 64) fail(WK_ERR_ARG, "FP32 engines need group_width 64");
-    if (rowptr_host[L->nrows] != L->nz) fail(WK_ERR_CSR, "rowptr does not match the layout");
+    if (flags & ~(uint32_t)(WK_ENGINE_KR | WK_ENGINE_ADJOINT | WK_ENGINE_ROWSORT))
+      fail(WK_ERR_ARG, "unknown engine flag");
+    // the kernels gather x[col]: every column must be in range
+    validate_csr(L->nrows, L->ncols, L->nz, rowptr_host, colind_host);
     const bool kr = flags & WK_ENGINE_KR, adj = flags & WK_ENGINE_ADJOINT;
-    if (kr && L->nrows != L->ncols) fail(WK_ERR_ARG, "the Kr form needs a square matrix");
+    const bool rowsort = flags & WK_ENGINE_ROWSORT;
+    if ((kr || rowsort) && L->nrows != L->ncols)
+      fail(WK_ERR_ARG, "the Kr form and the row sort need a square matrix");
     hipStream_t s = static_cast<hipStream_t>(stream);
     auto *E = new wk_engine();
     try {
@@ -219,7 +249,24 @@ int wk_engine_create(const wk_layout *L, const int32_t *rowptr_host, const int32
       E->ncols = L->ncols;
       E->nnz = L->nz;
       wk_layout_get_info(L, &E->info);
-      upload_layout(E, E->fwd, *L, vals_host, colind_host, values_dtype, kr, nullptr, s);
+      if (rowsort) {
+        // reorder_columns_rowsort + reorder_values_coalesced_rowsort (hpp:414-444,
+        // 505-515): each row's entries in permuted-column order; the CSR ->
+        // slot map follows every entry to its new slot
+        const int64_t nz = L->nz;
+        std::vector<int32_t> kcols((size_t)nz), src((size_t)nz), cols2((size_t)nz);
+        std::vector<double> vals2((size_t)nz);
+        std::vector<uint32_t> slot((size_t)nz);
+        reorder_columns_rowsort(*L, rowptr_host, colind_host, kcols.data(), src.data());
+        for (int64_t n = 0; n < nz; n++) {
+          vals2[n] = vals_host[src[n]];
+          cols2[n] = colind_host[src[n]];
+          slot[src[n]] = L->reorder_rows[n];
+        }
+        upload_layout(E, E->fwd, *L, vals2.data(), cols2.data(), values_dtype, kr, slot.data(), s);
+      } else {
+        upload_layout(E, E->fwd, *L, vals_host, colind_host, values_dtype, kr, nullptr, s);
+      }
       if (adj) {
         // A^T in CSR with the same scan options; its values are A's values
         // permuted, and slot_of_nz maps A's CSR index to the transposed slot.
@@ -286,6 +333,13 @@ int wk_engine_set_colinds(wk_engine *E, const int32_t *pcols, int32_t where, voi
     const size_t bytes = sizeof(int32_t) * (size_t)E->fwd.allocate_nz;
     if (!bytes) return;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    // the kernels gather x[col]: reject out-of-range columns before they land
+    if (where)
+      check_index_range_device(pcols, E->fwd.allocate_nz, E->ncols, s);
+    else
+      for (int64_t i = 0; i < E->fwd.allocate_nz; i++)
+        if ((uint32_t)pcols[i] >= (uint32_t)E->ncols)
+          fail(WK_ERR_ARG, "padded column out of range at slot " + std::to_string(i));
     hip_check(hipMemcpyAsync(E->fwd.cols, pcols, bytes,
                              where ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s),
               "set_colinds");

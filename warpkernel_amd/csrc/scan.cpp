@@ -10,6 +10,7 @@
 //     single forward pass that emits the sub-groups directly;
 //   - the reorder map and the padded scatter run on several host threads.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <functional>
 #include <thread>
@@ -43,9 +44,18 @@ void validate_csr(int32_t nrows, int32_t ncols, int64_t nnz, const int32_t *rowp
       fail(WK_ERR_CSR, "rowptr decreases at row " + std::to_string(r));
   if ((int64_t)rowptr[nrows] != nnz) fail(WK_ERR_CSR, "rowptr[nrows] != nnz");
   if (colind) {
-    for (int64_t i = 0; i < nnz; i++)
-      if (colind[i] < 0 || colind[i] >= ncols)
-        fail(WK_ERR_CSR, "column index out of range at entry " + std::to_string(i));
+    std::atomic<int64_t> first_bad{INT64_MAX};
+    parallel_for(nnz, 1 << 20, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; i++)
+        if ((uint32_t)colind[i] >= (uint32_t)ncols) {
+          int64_t cur = first_bad.load();
+          while (i < cur && !first_bad.compare_exchange_weak(cur, i)) {
+          }
+          return;
+        }
+    });
+    if (first_bad.load() != INT64_MAX)
+      fail(WK_ERR_CSR, "column index out of range at entry " + std::to_string(first_bad.load()));
   }
 }
 
@@ -253,6 +263,26 @@ template void process_host<double>(const wk_layout &, const double *, const int3
                                    int32_t *);
 template void process_host<float>(const wk_layout &, const double *, const int32_t *, float *,
                                   int32_t *);
+
+void reorder_columns_rowsort(const wk_layout &L, const int32_t *rowptr, const int32_t *colind,
+                             int32_t *cols_out, int32_t *src) {
+  // sort keys (permuted column << 32 | position in row) make the order total,
+  // which is std::sort by the column with CSR order kept among equal columns
+  parallel_for(L.nrows, 2048, [&](int64_t r0, int64_t r1) {
+    std::vector<uint64_t> key;
+    for (int64_t r = r0; r < r1; r++) {
+      const int32_t b = rowptr[r], n = rowptr[r + 1] - rowptr[r];
+      key.resize((size_t)n);
+      for (int32_t i = 0; i < n; i++)
+        key[i] = ((uint64_t)(uint32_t)L.row_map_inv[colind[b + i]] << 32) | (uint32_t)i;
+      std::sort(key.begin(), key.end());
+      for (int32_t i = 0; i < n; i++) {
+        cols_out[b + i] = (int32_t)(key[i] >> 32);
+        src[b + i] = b + (int32_t)(key[i] & 0xffffffffu);
+      }
+    }
+  });
+}
 
 void reorder_columns_coalesced(const wk_layout &L, const int32_t *colind, int32_t *out) {
   parallel_for(L.allocate_nz, 1 << 20, [&](int64_t a, int64_t b) {

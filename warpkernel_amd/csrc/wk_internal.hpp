@@ -28,6 +28,7 @@ struct wk_layout {
   std::vector<uint32_t> max_nz_rows;
   std::vector<uint32_t> threads_per_row;  // variant 2
   std::vector<uint32_t> row_offset_warp;  // variant 2
+  std::vector<int32_t> reorder_A_rows;    // after reorder_columns_rowsort
 };
 
 namespace wk {
@@ -52,6 +53,11 @@ template <class T>
 void process_host(const wk_layout &L, const double *vals, const int32_t *colind, T *pvals,
                   int32_t *pcols);
 void reorder_columns_coalesced(const wk_layout &L, const int32_t *colind, int32_t *out);
+// reorder_columns_rowsort (warpkernel.hpp:414-444): per row, entries ordered by
+// permuted column (ties in CSR order); cols_out[n] = permuted column,
+// src[n] = CSR index of the entry placed at n.  Needs a square matrix.
+void reorder_columns_rowsort(const wk_layout &L, const int32_t *rowptr, const int32_t *colind,
+                             int32_t *cols_out, int32_t *src);
 // CSR of A^T with the entries of each column in increasing row order;
 // perm[t] = CSR index of transposed entry t.
 void transpose_csr(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *colind,
@@ -129,6 +135,9 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
 void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
                            hipStream_t stream);
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
+// Fails with WK_ERR_ARG unless every a[i], i < n, lies in [0, bound); device
+// array, synchronises the stream.
+void check_index_range_device(const int32_t *a, int64_t n, int32_t bound, hipStream_t stream);
 // Row sums of the padded values (run_assembly); cfg supplies waves_per_block
 // and xcd_remap.
 void launch_rowsum(const DevLayout &D, int32_t dtype, const wk_launch &cfg, double *y,
