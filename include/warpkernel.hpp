@@ -111,6 +111,41 @@ class device_vector {
   size_t n_ = 0;
 };
 
+// The host CSR the drivers convert their MatrixMarket input to
+// (cusp::csr_matrix<int, double, host_memory>, wpk1.cu:48), same field names.
+struct csr_matrix {
+  uint num_rows = 0, num_cols = 0, num_entries = 0;
+  std::vector<int> row_offsets, column_indices;
+  std::vector<double> values;
+};
+
+// cusp::io::read_matrix_market_file + the COO -> CSR conversion
+// (wpk1.cu:43-48): coordinate files, entries ordered by (row, column).
+inline void read_matrix_market_file(csr_matrix &A, const char *path) {
+  wk_csr *h = nullptr;
+  check(wk_mm_read(path, &h));
+  int32_t nr = 0, nc = 0;
+  int64_t nnz = 0;
+  const int32_t *rp = nullptr, *ci = nullptr;
+  const double *va = nullptr;
+  const int rc = wk_csr_get(h, &nr, &nc, &nnz, &rp, &ci, &va);
+  if (rc == WK_OK) {
+    A.num_rows = (uint)nr;
+    A.num_cols = (uint)nc;
+    A.num_entries = (uint)nnz;
+    A.row_offsets.assign(rp, rp + nr + 1);
+    A.column_indices.assign(ci, ci + nnz);
+    A.values.assign(va, va + nnz);
+  }
+  wk_csr_destroy(h);
+  check(rc);
+}
+
+inline void write_matrix_market_file(const csr_matrix &A, const char *path) {
+  check(wk_mm_write(path, (int32_t)A.num_rows, (int32_t)A.num_cols, (int64_t)A.num_entries,
+                    A.row_offsets.data(), A.column_indices.data(), A.values.data()));
+}
+
 // WPK1 layout (hpp:215-551).
 class structure {
  public:
@@ -188,6 +223,27 @@ class structure {
     device_x = out;
   }
 
+  // structure::save / load (serialize.cu:135-139): the layout in a file (a
+  // versioned binary format of this library; load checks its invariants).
+  void save(const char *path) const { check(wk_layout_save(h_, path, driver_name.c_str())); }
+  void load(const char *path) {
+    wk_layout *h = nullptr;
+    char name[256];
+    check(wk_layout_load(path, &h, name, sizeof name));
+    wk_layout_info info;
+    const int rc = wk_layout_get_info(h, &info);
+    if (rc != WK_OK || info.variant != variant()) {
+      wk_layout_destroy(h);
+      throw error(rc != WK_OK ? rc : WK_ERR_ARG, "layout file holds the other structure variant");
+    }
+    group_width_ = info.group_width;
+    driver_name = name;
+    adopt(h);
+    rows_ = array<int>(WK_ROWPTR);
+    reorder_A_rows = array<int>(WK_REORDER_A_ROWS);
+  }
+  std::string driver_name;  // hpp:218, stored with the layout
+
   const wk_layout *handle() const { return h_; }
   const std::vector<int> &rowptr() const { return rows_; }
   int group_width() const { return group_width_; }
@@ -201,6 +257,7 @@ class structure {
     const T *p = static_cast<const T *>(d);
     return std::vector<T>(p, p + n);
   }
+  virtual int variant() const { return 1; }
   virtual void adopt(wk_layout *h) {
     wk_layout_destroy(h_);
     h_ = h;
@@ -250,6 +307,7 @@ class structure2 : public structure {
   }
 
  protected:
+  int variant() const override { return 2; }
   void adopt(wk_layout *h) override {
     structure::adopt(h);
     wk_layout_info info;

@@ -60,6 +60,7 @@ extern "C" {
 #define WK_THREADS_PER_ROW 5 /* uint32 [nwarps]    structure2 only                hpp:562 */
 #define WK_ROW_OFFSET_WARP 6 /* uint32 [nwarps+1]  structure2 only                hpp:561 */
 #define WK_REORDER_A_ROWS 7  /* int32 [nz] after wk_layout_reorder_columns_rowsort  hpp:279 */
+#define WK_ROWPTR 8          /* int32 [nrows+1]    the scanned CSR's row offsets */
 
 /* wk_engine_create flags */
 #define WK_ENGINE_KR 1u       /* also build the Kr (run_x) column indices */
@@ -138,6 +139,35 @@ int wk_layout_reorder_values_rowsort(const wk_layout *layout, const double *vals
 
 void wk_layout_destroy(wk_layout *layout);
 
+/* ------------------------------------------------------------------- files */
+
+/* A host CSR matrix read from a file (int32 indices, FP64 values). */
+typedef struct wk_csr wk_csr;
+
+/* cusp::io::read_matrix_market_file + the COO -> CSR conversion of the drivers
+ * (wpk1.cu:43-48, wpk2.cu:55-59).  Coordinate format; field real, double,
+ * integer or pattern (entries 1); symmetry general, symmetric, skew-symmetric
+ * (mirrored entries negated) or hermitian (real: as symmetric).  Entries are
+ * ordered by (row, column), stably; duplicates are kept. */
+int wk_mm_read(const char *path, wk_csr **out);
+/* Borrowed views (valid until wk_csr_destroy); any output may be NULL. */
+int wk_csr_get(const wk_csr *csr, int32_t *nrows, int32_t *ncols, int64_t *nnz,
+               const int32_t **rowptr, const int32_t **colind, const double **vals);
+void wk_csr_destroy(wk_csr *csr);
+/* Writes "coordinate real general", entries in CSR order, each value as the
+ * shortest decimal that reads back to the same double. */
+int wk_mm_write(const char *path, int32_t nrows, int32_t ncols, int64_t nnz,
+                const int32_t *rowptr_host, const int32_t *colind_host, const double *vals_host);
+
+/* structure::save / load (serialize.cu:135-139) over the fields of
+ * structure::serialize (warpkernel.hpp:252-270) plus structure2's: a versioned
+ * little-endian binary file with a checksum (the reference's boost archive is
+ * not reproduced).  load checks the layout's invariants (WK_ERR_CSR on a
+ * damaged file); driver_name (may be NULL) receives the stored name, cut to
+ * name_capacity - 1 bytes. */
+int wk_layout_save(const wk_layout *layout, const char *path, const char *driver_name);
+int wk_layout_load(const char *path, wk_layout **out, char *driver_name, int64_t name_capacity);
+
 /* ------------------------------------------------------------------ engine */
 
 typedef struct wk_engine wk_engine;
@@ -146,7 +176,8 @@ typedef struct wk_engine wk_engine;
  * (warpkernel.hpp:794-809) -> structure::process (hpp:532-548) and
  * structure2::process (hpp:749-769): the padded arrays and the layout metadata
  * are built and copied to the current device.  vals_host is FP64 CSR (cast to
- * FP32 when values_dtype is WK_F32).  The engine copies what it needs; the
+ * FP32 when values_dtype is WK_F32); rowptr_host may be NULL (the layout's
+ * own offsets, WK_ROWPTR).  The engine copies what it needs; the
  * layout may be destroyed afterwards. */
 int wk_engine_create(const wk_layout *layout, const int32_t *rowptr_host,
                      const int32_t *colind_host, const double *vals_host, int32_t values_dtype,

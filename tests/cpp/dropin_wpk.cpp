@@ -2,8 +2,10 @@
 // written against include/warpkernel.hpp, checked against the oracle.
 //   dropin_wpk [n]   3D hex elasticity on an n^3 element grid (default 16)
 // Exit status 0 when every result matches; 1 otherwise.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "../../include/warpkernel.hpp"
@@ -88,6 +90,72 @@ int main(int argc, char **argv) {
       }
       expect(std::vector<int>(eng.device_colinds) == restore_col, "colinds restored");
     }
+  }
+
+  // MatrixMarket in, layout saved and loaded again (wpk1.cu:43-48, serialize.cu:130-160)
+  {
+    const char *tmp = std::getenv("TMPDIR");
+    const std::string dir = tmp && *tmp ? tmp : "/tmp";
+    const std::string mtx = dir + "/wk_dropin_" + std::to_string(n) + ".mtx";
+    const std::string lay = dir + "/wk_dropin_" + std::to_string(n) + ".k1";
+    warpkernel::csr_matrix A;
+    A.num_rows = M;
+    A.num_cols = N;
+    A.num_entries = nz;
+    A.row_offsets = rowptr;
+    A.column_indices = colind;
+    A.values = vals;
+    warpkernel::write_matrix_market_file(A, mtx.c_str());
+    warpkernel::csr_matrix B;
+    warpkernel::read_matrix_market_file(B, mtx.c_str());
+    expect(B.num_rows == M && B.row_offsets == rowptr && B.column_indices == colind && B.values == vals,
+           "MatrixMarket round trip");
+    warpkernel::structure kernel1;
+    kernel1.driver_name = "dropin";
+    kernel1.scan(B.num_entries, B.num_rows, B.values.data(), B.row_offsets.data(), B.column_indices.data());
+    kernel1.save(lay.c_str());
+    warpkernel::structure kernel2;
+    kernel2.load(lay.c_str());
+    expect(kernel2.reorder_rows == kernel1.reorder_rows && kernel2.row_map == kernel1.row_map &&
+               kernel2.warp_offsets == kernel1.warp_offsets && kernel2.driver_name == "dropin",
+           "layout file round trip");
+    warpkernel::engine<double, int> eng(kernel2, B.values.data(), B.column_indices.data());
+    warpkernel::device_vector<double> dy(N, 0.);
+    eng.run<true>((kernel2.nwarps + 3) / 4, 256, dx.data(), dy.data());
+    expect(std::vector<double>(dy) == y, "engine from a loaded layout bit-exact");
+    // run_assembly (hpp:898-929): row sums, the CSR sum at x = 1
+    std::vector<double> ones(N, 1.0), rs(M);
+    wko_csr_spmv((int32_t)M, rowptr.data(), colind.data(), vals.data(), ones.data(), rs.data(), 1, 1, 1, 1);
+    warpkernel::device_vector<double> ds(N, 0.);
+    eng.run_assembly<true>((kernel2.nwarps + 3) / 4, 256, ds.data());
+    expect(std::vector<double>(ds) == rs, "run_assembly bit-exact");
+    std::remove(mtx.c_str());
+    std::remove(lay.c_str());
+  }
+
+  // Kr row sort (optimize.cu:645-704): each row in permuted-column order
+  {
+    warpkernel::structure kernel1;
+    kernel1.scan(nz, N, vals.data(), rowptr.data(), colind.data());
+    std::vector<int> reorder_cols = colind;
+    kernel1.reorder_columns_rowsort(reorder_cols, rowptr);
+    warpkernel::device_vector<double> dreordered_x;
+    kernel1.reorder_x(x, dreordered_x);
+    std::vector<double> A_new_values(nz);
+    std::vector<int> cols_sorted(nz);
+    for (uint i = 0; i < nz; i++) {
+      A_new_values[i] = vals[kernel1.reorder_A_rows[i]];
+      cols_sorted[i] = colind[kernel1.reorder_A_rows[i]];
+    }
+    warpkernel::engine<double, int> eng(kernel1, A_new_values.data(), reorder_cols.data());
+    warpkernel::device_vector<double> dy(N, 0.);
+    eng.run_x<true>((kernel1.nwarps + 3) / 4, 256, dreordered_x.data(), dy.data());
+    std::vector<double> ycheck = dy, ys(M);
+    wko_csr_spmv((int32_t)M, rowptr.data(), cols_sorted.data(), A_new_values.data(), x.data(), ys.data(), 1, 1, 1, 1);
+    bool same = true;
+    for (uint i = 0; i < M; i++) same = same && ycheck[i] == ys[kernel1.row_map[i]];
+    expect(same, "rowsort run_x bit-exact vs the row-sorted CSR order");
+    expect(eng.verify_x(y, ycheck), "rowsort verify_x");
   }
 
   // WPK2 threshold sweep (wpk2.cu:140-157)

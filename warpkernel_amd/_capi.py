@@ -17,6 +17,7 @@ WK_F64, WK_F32 = 0, 1
 WK_ACC_TWOSUM, WK_ACC_F64 = 0, 1
 WK_ROW_MAP, WK_ROW_MAP_INV, WK_REORDER_ROWS, WK_WARP_OFFSETS = 0, 1, 2, 3
 WK_MAX_NZ_ROWS, WK_THREADS_PER_ROW, WK_ROW_OFFSET_WARP, WK_REORDER_A_ROWS = 4, 5, 6, 7
+WK_ROWPTR = 8
 WK_ENGINE_KR, WK_ENGINE_ADJOINT, WK_ENGINE_ROWSORT = 1, 2, 4
 WK_ENGINE_VALUES, WK_ENGINE_COLS, WK_ENGINE_COLS_KR = 16, 17, 18
 
@@ -66,6 +67,13 @@ SIGNATURES = {
     "wk_layout_reorder_x": (C.c_int, [P, P, P]),
     "wk_layout_reorder_columns_rowsort": (C.c_int, [P, P, P, P]),
     "wk_layout_reorder_values_rowsort": (C.c_int, [P, P, P]),
+    "wk_mm_read": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "wk_csr_get": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(I64), C.POINTER(P),
+                             C.POINTER(P), C.POINTER(P)]),
+    "wk_csr_destroy": (None, [P]),
+    "wk_mm_write": (C.c_int, [C.c_char_p, I32, I32, I64, P, P, P]),
+    "wk_layout_save": (C.c_int, [P, C.c_char_p, C.c_char_p]),
+    "wk_layout_load": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, I64]),
     "wk_layout_destroy": (None, [P]),
     "wk_engine_create": (C.c_int, [P, P, P, P, I32, U32, P, PP]),
     "wk_engine_destroy": (None, [P]),

@@ -151,6 +151,11 @@ class structure:
         check(lib().wk_layout_reorder_x(self._h, _host_ptr(x), _host_ptr(out)))
         return out
 
+    def save(self, path, driver_name: str = ""):
+        """structure::save (serialize.cu:135): the layout to a file."""
+        from .io import save_structure
+        save_structure(self, path, driver_name)
+
     def close(self):
         if self._h is not None:
             lib().wk_layout_destroy(self._h)

@@ -168,6 +168,7 @@ int wk_layout_array(const wk_layout *L, int32_t which, const void **data, int64_
         *data = L->row_offset_warp.data(); *count = (int64_t)L->row_offset_warp.size(); break;
       case WK_REORDER_A_ROWS:
         *data = L->reorder_A_rows.data(); *count = (int64_t)L->reorder_A_rows.size(); break;
+      case WK_ROWPTR: *data = L->rowptr.data(); *count = (int64_t)L->rowptr.size(); break;
       default: fail(WK_ERR_ARG, "unknown layout array");
     }
   });
@@ -229,7 +230,11 @@ int wk_engine_create(const wk_layout *L, const int32_t *rowptr_host, const int32
   return guarded([&] {
     if (!out) fail(WK_ERR_ARG, "out is NULL");
     *out = nullptr;
-    if (!L || !rowptr_host || !colind_host || !vals_host) fail(WK_ERR_ARG, "NULL argument");
+    if (!L || (L->nz && (!colind_host || !vals_host))) fail(WK_ERR_ARG, "NULL argument");
+    // rowptr_host NULL: the offsets the layout was scanned (or loaded) with
+    if (!rowptr_host) rowptr_host = L->rowptr.data();
+    if ((int64_t)L->rowptr.size() != (int64_t)L->nrows + 1 && rowptr_host == L->rowptr.data())
+      fail(WK_ERR_ARG, "rowptr is NULL and the layout holds none");
     if (values_dtype != WK_F64 && values_dtype != WK_F32) fail(WK_ERR_ARG, "unknown values dtype");
     if (values_dtype == WK_F32 && L->W != 64) fail(WK_ERR_ARG, "FP32 engines need group_width 64");
     if (flags & ~(uint32_t)(WK_ENGINE_KR | WK_ENGINE_ADJOINT | WK_ENGINE_ROWSORT))

@@ -115,6 +115,7 @@ void scan1(wk_layout &L, int32_t W, int32_t nrows, int32_t ncols, int64_t nnz,
   L.nrows = nrows;
   L.ncols = ncols;
   L.nz = nnz;
+  L.rowptr.assign(rowptr, rowptr + (size_t)nrows + 1);
   std::vector<uint32_t> slen;
   sort_rows(L, rowptr, slen);
   L.nwarps = (L.nznrows + W - 1) / W;
@@ -199,6 +200,7 @@ void scan2(wk_layout &L, int32_t W, int32_t threshold, int32_t max_tpr, int32_t 
   L.nrows = nrows;
   L.ncols = ncols;
   L.nz = nnz;
+  L.rowptr.assign(rowptr, rowptr + (size_t)nrows + 1);
   std::vector<uint32_t> slen;
   sort_rows(L, rowptr, slen);
   const int32_t nzr = L.nznrows;

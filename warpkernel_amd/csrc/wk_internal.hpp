@@ -29,6 +29,7 @@ struct wk_layout {
   std::vector<uint32_t> threads_per_row;  // variant 2
   std::vector<uint32_t> row_offset_warp;  // variant 2
   std::vector<int32_t> reorder_A_rows;    // after reorder_columns_rowsort
+  std::vector<int32_t> rowptr;            // the scanned CSR's row offsets [nrows+1]
 };
 
 namespace wk {
