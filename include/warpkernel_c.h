@@ -313,18 +313,19 @@ int wk_engine_set_launch(wk_engine *engine, int32_t op, int32_t k, const wk_laun
 
 /* ------------------------------------------------------ batches and graphs */
 
-/* Many matrices in one persistent launch (no reference counterpart; the
- * reference launches one kernel per matrix and per call).  The groups of all
- * engines (and the chunks of their empty rows) form one work list, split into
- * contiguous ranges of equal cost over a fixed grid of persistent waves;
- * neighbouring groups stay on one wave, no atomics.  Engines must share the
- * value type, use group width 64 and outlive the batch. */
+/* Many matrices in one launch (no reference counterpart; the reference
+ * launches one kernel per matrix and per call).  The groups of all engines
+ * (and the chunks of their empty rows) form one work list, longest groups
+ * first; no atomics.  Engines must share the value type, use group width 64
+ * and outlive the batch. */
 typedef struct wk_batch wk_batch;
 
-/* waves_per_cu: persistent waves per compute unit (0: 8).  stage_lds: as in
- * wk_launch, for batches with one right-hand side (0 direct loads, 1 staged,
- * 2 staged with the next batch in flight); applied to the matrices whose mean
- * group length is at least 24, the others use direct loads. */
+/* waves_per_cu: 0 (the default) runs one wave per work item and lets the
+ * dispatcher balance them; > 0 (at most 1024) runs that many persistent waves
+ * per compute unit, each over a contiguous range of equal cost.  stage_lds: as
+ * in wk_launch, for batches with one right-hand side (0 direct loads, 1
+ * staged, 2 staged with the next batch in flight); applied to the matrices
+ * whose mean group length is at least 24, the others use direct loads. */
 int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
                     int32_t stage_lds, wk_batch **out);
 

@@ -1,4 +1,5 @@
-"""One persistent launch over many uneven FE matrices, and HIP graph replay."""
+"""One launch over many uneven FE matrices (one wave per group, or persistent
+waves over equal-cost ranges), and HIP graph replay."""
 import sys
 
 import numpy as np
@@ -49,17 +50,19 @@ def make_engines(ms, dtype="f64"):
     return engs
 
 
-@pytest.mark.parametrize("k,stage", [(1, 0), (1, 1), (1, 2), (4, 2), (3, 0)])
-def test_batch_of_64_matrices_bitexact(k, stage):
+@pytest.mark.parametrize("k,stage,wpc", [(1, 0, 0), (1, 1, 0), (1, 2, 0), (4, 2, 0), (3, 0, 0),
+                                         (1, 2, 8), (1, 0, 64), (4, 2, 16)])
+def test_batch_of_64_matrices_bitexact(k, stage, wpc):
     ms = mix()
-    say(f"[gpu] batch of {len(ms)} matrices, {sum(m.nnz for m in ms)} nnz, k={k}, stage={stage}")
+    say(f"[gpu] batch of {len(ms)} matrices, {sum(m.nnz for m in ms)} nnz, k={k}, stage={stage}, "
+        f"waves_per_cu={wpc}")
     engs = make_engines(ms)
     xs_h = [fe.uniform_x(m.ncols, k, seed=7000 + i).reshape((m.ncols,) if k == 1 else (m.ncols, k))
             for i, m in enumerate(ms)]
     xs = [dev(x) for x in xs_h]
     ys = [torch.full((m.nrows,) if k == 1 else (m.nrows, k), float("nan"), dtype=torch.float64,
                      device="cuda") for m in ms]
-    b = batch(engs, stage_lds=stage)
+    b = batch(engs, waves_per_cu=wpc, stage_lds=stage)
     b.bind(xs, ys)
     b.run()
     torch.cuda.synchronize()

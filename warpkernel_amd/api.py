@@ -470,11 +470,13 @@ def _verify(orig, comp, row_map) -> bool:
 
 
 class batch:
-    """Many engines in one persistent launch (wk_batch_*).
+    """Many engines in one launch (wk_batch_*).
 
         b = batch([eng0, eng1, ...]);  b.bind(xs, ys);  b.run()
 
-    xs / ys: lists of device tensors, [ncols_i] / [nrows_i] or [n, k]."""
+    xs / ys: lists of device tensors, [ncols_i] / [nrows_i] or [n, k].
+    waves_per_cu = 0: one wave per group (default); > 0: that many persistent
+    waves per compute unit over equal-cost ranges."""
 
     def __init__(self, engines, waves_per_cu: int = 0, stage_lds: int = 2):
         self.engines = list(engines)  # keep them alive
