@@ -180,8 +180,14 @@ def extra_fwd_adjoint(torch, dev, steps, warmup, tune=True):
         _, f = time_launches(torch, lambda: eng.launch(x, y), steps, warmup)
         _, a = time_launches(torch, lambda: eng.run_adjoint(dyv, dxv), steps, warmup)
         _, d = time_launches(torch, lambda: eng.dvals(dyv, x, dv), steps, warmup)
+        # the backward as autograd runs it: dx then dvals in one call
+        _, bw = time_launches(torch, lambda: eng.backward(dyv, x, dxv, dv), steps, warmup)
+        # one training step's SpMV work back to back: forward, then backward
+        _, seq = time_launches(torch, lambda: (eng.launch(x, y), eng.backward(dyv, x, dxv, dv)),
+                               steps, warmup)
         fa[f"k{k}"] = {"fwd_ms": round(f, 4), "dx_ms": round(a, 4), "dvals_ms": round(d, 4),
-                       "fwd_adjoint_ms": round(f + a + d, 4),
+                       "backward_ms": round(bw, 4), "fwd_adjoint_ms": round(seq, 4),
+                       "fwd_plus_separate_dx_dvals_ms": round(f + a + d, 4),
                        "fwd_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (f * 1e-3) / 1e9, 1)}
     out["fwd_adjoint_hex_laplace_3d_48"] = fa
     del eng

@@ -266,6 +266,13 @@ int wk_residual(wk_engine *engine, const wk_launch *cfg, const double *b_dev, in
                 const double *x_dev, double *r_dev, int32_t k, int64_t ldx, int64_t ldr,
                 void *stream);
 
+/* The backward of y = A(vals) x in one call: dx = A^T dy (as wk_run_adjoint,
+ * with cfg) then dvals (as wk_dvals) on `stream`, FP64 engines.  No reference
+ * counterpart (the reference has no adjoint). */
+int wk_backward(wk_engine *engine, const wk_launch *cfg, const void *dy_dev, int64_t lddy,
+                const void *x_dev, int64_t ldx, double *dx_dev, int64_t lddx, double *dvals_dev,
+                int32_t k, void *stream);
+
 /* engine::run_assembly<cache> (warpkernel.hpp:898-929, kernel hpp:78-100):
  * y[row_map[p]] = the padded values of sorted row p summed in slot order
  * (first value, then one add per slot, pads included) in the engine's value

@@ -99,3 +99,37 @@ def test_values_change_between_steps():
         assert np.array_equal(y.detach().cpu().numpy(), ref)
         (y ** 2).sum().backward()
         opt.step()
+
+
+@pytest.mark.parametrize("variant,th", [(1, 0), (2, 3)])
+@pytest.mark.parametrize("case", ["hex_laplace_3d_48", "tet_poisson_3d_96", "random_empty_rows"])
+def test_backward_equals_adjoint_plus_dvals(case, variant, th):
+    """wk_backward against run_adjoint + dvals, bit for bit, and dvals against
+    the oracle."""
+    from conftest import random_csr
+    from warpkernel_amd import engine, structure, structure2
+    say(f"[gpu] backward {case} variant={variant}")
+    if case == "random_empty_rows":
+        rp, cols, vals = random_csr(np.random.default_rng(5), 5000, 4000, 1)
+        m = fe.CSR(case, 5000, 4000, rp, cols, vals)
+    else:
+        m = fe.CONFIGS[case]()
+    s = (structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, ncols=m.ncols) if variant == 1 else
+         structure2(64, 32).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, th, ncols=m.ncols))
+    eng = engine(s, m.vals, m.colind, adjoint=True)
+    rng = np.random.default_rng(6)
+    for k in (1, 8):
+        shp = lambda n: (n,) if k == 1 else (n, k)  # noqa: E731
+        x = torch.from_numpy(rng.uniform(-1, 1, shp(m.ncols))).cuda()
+        dy = torch.from_numpy(rng.uniform(-1, 1, shp(m.nrows))).cuda()
+        dx1, dx2 = (torch.full(shp(m.ncols), float("nan"), dtype=torch.float64, device="cuda") for _ in range(2))
+        dv1, dv2 = (torch.full((m.nnz,), float("nan"), dtype=torch.float64, device="cuda") for _ in range(2))
+        eng.backward(dy, x, dx1, dv1)
+        eng.run_adjoint(dy, dx2)
+        eng.dvals(dy, x, dv2)
+        torch.cuda.synchronize()
+        assert torch.equal(dx1, dx2), k
+        assert torch.equal(dv1, dv2), k
+        if k == 1:
+            assert np.array_equal(dv1.cpu().numpy(), O.dvals(m.rowptr, m.colind, dy.cpu().numpy(), x.cpu().numpy()))
+    eng.close()

@@ -88,6 +88,7 @@ SIGNATURES = {
     "wk_run": (C.c_int, [P, C.POINTER(Launch), P, P, I32, I64, I64, P]),
     "wk_run_x": (C.c_int, [P, C.POINTER(Launch), P, P, I32, I64, I64, P]),
     "wk_run_assembly": (C.c_int, [P, C.POINTER(Launch), P, P]),
+    "wk_backward": (C.c_int, [P, C.POINTER(Launch), P, I64, P, I64, P, I64, P, I32, P]),
     "wk_residual": (C.c_int, [P, C.POINTER(Launch), P, I64, P, P, I32, I64, I64, P]),
     "wk_run_adjoint": (C.c_int, [P, C.POINTER(Launch), P, P, I32, I64, I64, P]),
     "wk_dvals": (C.c_int, [P, P, I64, P, I64, I32, P, P]),

@@ -385,6 +385,15 @@ class engine:
         _, ldx = self._k_ld(x)
         check(lib().wk_dvals(self._h, _dev_ptr(dy), lddy, _dev_ptr(x), ldx, k, _dev_ptr(out), _stream(stream)))
 
+    def backward(self, dy, x, dx, dvals, cfg=None, stream=None):
+        """dx = A^T dy and dvals in one call (FP64 engines)."""
+        k, lddy = self._k_ld(dy)
+        _, ldx = self._k_ld(x)
+        _, lddx = self._k_ld(dx)
+        c = self._cfg(capi.WK_OP_ADJOINT, k, cfg)
+        check(lib().wk_backward(self._h, self._ref(c), _dev_ptr(dy), lddy, _dev_ptr(x), ldx, _dev_ptr(dx),
+                                lddx, _dev_ptr(dvals), k, _stream(stream)))
+
     def update_values(self, vals_dev, stream=None):
         """GPU-side process of new CSR values (same pattern)."""
         check(lib().wk_engine_update_values(self._h, _dev_ptr(vals_dev), _stream(stream)))

@@ -73,6 +73,13 @@ class WpkSpMV(torch.autograd.Function):
         op: WpkOperator = ctx.op
         gy = gy.contiguous()
         dvals = dx = None
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
+            if op._synced != ctx.vals_key:
+                raise RuntimeError("the operator's values changed between forward and backward")
+            dvals = torch.empty(op.nnz, dtype=torch.float64, device=gy.device)
+            dx = torch.empty_like(x)
+            op.engine.backward(gy, x, dx, dvals)
+            return dvals, dx, None
         if ctx.needs_input_grad[0]:
             dvals = torch.empty(op.nnz, dtype=torch.float64, device=gy.device)
             op.engine.dvals(gy, x, dvals)

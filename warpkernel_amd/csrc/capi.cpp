@@ -435,6 +435,25 @@ int wk_dvals(wk_engine *E, const double *dy_dev, int64_t lddy, const double *x_d
   });
 }
 
+int wk_backward(wk_engine *E, const wk_launch *cfg, const void *dy_dev, int64_t lddy,
+                const void *x_dev, int64_t ldx, double *dx_dev, int64_t lddx, double *dvals_dev,
+                int32_t k, void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (!E->has_adj) fail(WK_ERR_STATE, "engine was created without WK_ENGINE_ADJOINT");
+    if (k < 1 || lddy < k || ldx < k || lddx < k) fail(WK_ERR_ARG, "bad k or leading dimension");
+    if ((!dy_dev && E->nrows) || (!dx_dev && E->ncols) || (E->nnz && (!x_dev || !dvals_dev)))
+      fail(WK_ERR_ARG, "NULL vector");
+    if (E->dtype != WK_F64) fail(WK_ERR_ARG, "the combined backward is computed by FP64 engines");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const wk_launch c = resolve(E, WK_OP_ADJOINT, k, cfg);
+    launch_spmv(E->adj, E->dtype, c, OUT_RUN, dy_dev, lddy, dx_dev, lddx, nullptr, 0, k, s);
+    launch_dvals(E->d_rowptr, E->d_colind, E->d_tile_row, E->nrows, E->nnz,
+                 static_cast<const double *>(dy_dev), lddy, static_cast<const double *>(x_dev), ldx,
+                 k, dvals_dev, s);
+  });
+}
+
 int wk_run_assembly(wk_engine *E, const wk_launch *cfg, double *y_dev, void *stream) {
   return guarded([&] {
     check_engine(E);

@@ -179,24 +179,23 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_kernel(SpmvArgs a, int32_t n
 // ceiling.  Only the wave touches its slice, so no barrier is needed: LDS
 // operations of one wave complete in order.
 template <class P, bool SPLIT, int OUT, int U, bool PIPE>
-__global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, int32_t nwork,
-                                                                  int32_t remap) {
+__device__ __forceinline__ void staged_block(const SpmvArgs &a, int32_t bid, int32_t nwork,
+                                             int32_t remap, char *lds) {
   using V = typename P::V;
   using X = typename P::X;
   using Acc = typename P::Acc;
   constexpr int W = 64;
-  extern __shared__ __attribute__((aligned(16))) char lds[];
   const int32_t lane = threadIdx.x & 63;
   const int32_t wpb = blockDim.x >> 6;
-  if ((int32_t)blockIdx.x >= nwork) {
-    const int64_t pos = (int64_t)a.nznrows + (int64_t)(blockIdx.x - nwork) * blockDim.x + threadIdx.x;
+  if (bid >= nwork) {
+    const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
     if (pos >= a.nrows) return;
     const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
     a.y[dst * a.ldy] = OUT == OUT_RESID ? a.b[dst * a.ldb] : 0.0;
     return;
   }
   const int32_t wib = threadIdx.x >> 6;
-  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(blockIdx.x, nwork, remap) * wpb + wib);
+  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(bid, nwork, remap) * wpb + wib);
   if (wave >= a.ngroups) return;
   const GroupMeta m = load_meta<W, SPLIT>(a, wave);
   const V *vbase = static_cast<const V *>(a.vals) + m.base;
@@ -222,6 +221,13 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, in
     a.y[dst * a.ldy] = res;
 }
 
+template <class P, bool SPLIT, int OUT, int U, bool PIPE>
+__global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, int32_t nwork,
+                                                                  int32_t remap) {
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  staged_block<P, SPLIT, OUT, U, PIPE>(a, (int32_t)blockIdx.x, nwork, remap, lds);
+}
+
 // ------------------------------------------------------------ small kernels
 __global__ void fill_zero_kernel(uint32_t *p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -242,62 +248,75 @@ __global__ void scatter_kernel(const T *__restrict__ src, const uint32_t *__rest
 // the row holding the tile's first entry.  The block's threads walk the rows
 // the tile spans and stamp each entry slot of an LDS array with its row, so
 // every entry then finds its row with one conflict-free LDS read.  Column
-// reads and dvals writes are coalesced.
+// reads and dvals writes are coalesced.  The loads that do not need the rows
+// (columns, and x for k = 1) are issued before the stamping, so their latency
+// overlaps the tile_row -> rowptr chain.
 constexpr int kDvTile = 2048;
+constexpr int kDvThreads = 256;
 
-__global__ __launch_bounds__(256) void dvals_kernel(const int32_t *__restrict__ rowptr,
-                                                    const int32_t *__restrict__ colind,
-                                                    const int32_t *__restrict__ tile_row,
-                                                    int32_t nrows, int64_t nnz,
-                                                    const double *__restrict__ dy, int64_t lddy,
-                                                    const double *__restrict__ x, int64_t ldx,
-                                                    int32_t k, double *__restrict__ dv) {
-  __shared__ int32_t srow[kDvTile];
-  const int64_t e0 = (int64_t)blockIdx.x * kDvTile;
-  const int64_t e1 = e0 + kDvTile < nnz ? e0 + kDvTile : nnz;
-  // rows of the tile lie in [r0, r1]: the next tile starts in row r1 or later
-  const int32_t r0 = tile_row[blockIdx.x];
-  const int32_t r1 = tile_row[blockIdx.x + 1];
-  for (int32_t r = r0 + (int32_t)threadIdx.x; r <= r1; r += (int32_t)blockDim.x) {
-    const int64_t a = rowptr[r] > e0 ? rowptr[r] : e0;
-    const int64_t b = rowptr[r + 1] < e1 ? rowptr[r + 1] : e1;
-    for (int64_t e = a; e < b; e++) srow[e - e0] = r;
-  }
-  __syncthreads();
-  // all entries of a thread at once: rows, indices, then gathers, so their
-  // memory latencies overlap instead of adding up
-  constexpr int PER = kDvTile / 256;
-  int32_t r[PER], c[PER];
+struct DvArgs {
+  const int32_t *rowptr, *colind, *tile_row;
+  int64_t nnz;
+  const double *dy;
+  int64_t lddy;
+  const double *x;
+  int64_t ldx;
+  int32_t k;
+  double *dv;
+};
+
+__device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_t *srow) {
+  constexpr int PER = kDvTile / kDvThreads;
+  const int64_t e0 = (int64_t)tile * kDvTile;
+  const int64_t e1 = e0 + kDvTile < d.nnz ? e0 + kDvTile : d.nnz;
+  const int32_t r0 = d.tile_row[tile];
+  const int32_t r1 = d.tile_row[tile + 1];
+  int32_t c[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int64_t e = e0 + threadIdx.x + (int64_t)q * blockDim.x;
-    r[q] = e < e1 ? srow[e - e0] : r0;
-    c[q] = e < e1 ? colind[e] : 0;
+    const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
+    c[q] = e < e1 ? d.colind[e] : 0;
   }
-  if (k == 1) {
-    double xv[PER], dvv[PER];
+  // rows of the tile lie in [r0, r1]: the next tile starts in row r1 or later
+  for (int32_t r = r0 + (int32_t)threadIdx.x; r <= r1; r += kDvThreads) {
+    const int64_t a = d.rowptr[r] > e0 ? d.rowptr[r] : e0;
+    const int64_t b = d.rowptr[r + 1] < e1 ? d.rowptr[r + 1] : e1;
+    for (int64_t e = a; e < b; e++) srow[e - e0] = r;
+  }
+  if (d.k == 1) {
+    double xv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) xv[q] = d.x[(int64_t)c[q] * d.ldx];
+    __syncthreads();
+    double dvv[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      xv[q] = x[(int64_t)c[q] * ldx];
-      dvv[q] = dy[(int64_t)r[q] * lddy];
+      const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
+      dvv[q] = d.dy[(int64_t)(e < e1 ? srow[e - e0] : r0) * d.lddy];
     }
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      const int64_t e = e0 + threadIdx.x + (int64_t)q * blockDim.x;
-      if (e < e1) dv[e] = dvv[q] * xv[q];
+      const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
+      if (e < e1) d.dv[e] = dvv[q] * xv[q];
     }
     return;
   }
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int64_t e = e0 + threadIdx.x + (int64_t)q * blockDim.x;
+    const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
     if (e >= e1) continue;
-    const double *xr = x + (int64_t)c[q] * ldx;
-    const double *dr = dy + (int64_t)r[q] * lddy;
+    const double *xr = d.x + (int64_t)c[q] * d.ldx;
+    const double *dr = d.dy + (int64_t)srow[e - e0] * d.lddy;
     double s = dr[0] * xr[0];
-    for (int32_t j = 1; j < k; j++) s = __builtin_fma(dr[j], xr[j], s);
-    dv[e] = s;
+    for (int32_t j = 1; j < d.k; j++) s = __builtin_fma(dr[j], xr[j], s);
+    d.dv[e] = s;
   }
+}
+
+__global__ __launch_bounds__(kDvThreads) void dvals_kernel(DvArgs d) {
+  __shared__ int32_t srow[kDvTile];
+  dvals_tile(d, (int32_t)blockIdx.x, srow);
 }
 
 // ------------------------------------------------------------- dispatch
@@ -544,11 +563,12 @@ void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const
 void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *tile_row,
                   int32_t nrows, int64_t nnz, const double *dy, int64_t lddy, const double *x,
                   int64_t ldx, int32_t k, double *dvals, hipStream_t stream) {
+  (void)nrows;
   if (nnz <= 0) return;
   const int64_t blocks = (nnz + kDvTile - 1) / kDvTile;
   if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
-  hipLaunchKernelGGL(dvals_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, rowptr, colind,
-                     tile_row, nrows, nnz, dy, lddy, x, ldx, k, dvals);
+  const DvArgs d{rowptr, colind, tile_row, nnz, dy, lddy, x, ldx, k, dvals};
+  hipLaunchKernelGGL(dvals_kernel, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
   hip_check(hipGetLastError(), "dvals launch");
 }
 
