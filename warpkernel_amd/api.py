@@ -41,8 +41,12 @@ def _dev_ptr(t) -> int:
 
 
 def _stream(stream) -> int:
+    """The raw hipStream_t: the given stream, or torch's current stream."""
     if stream is None:
         import torch
+        raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        if raw is not None:  # 0.1 us instead of 3 us for current_stream().cuda_stream
+            return int(raw(torch.cuda.current_device()))
         return int(torch.cuda.current_stream().cuda_stream)
     if isinstance(stream, int):
         return stream

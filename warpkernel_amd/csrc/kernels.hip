@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "spmv_device.hpp"
@@ -423,6 +424,29 @@ StagedFn pick_staged(int prec, bool split, int out, int u, bool pipe) {
   return pick_staged_split<PF32D>(split, out, u, pipe);
 }
 
+// LDS limits of the current device, read once per device (launches are on the
+// host's hot path: a kernel of a few microseconds must not wait on queries).
+struct LdsLimits {
+  int per_cu, per_block;
+};
+
+LdsLimits lds_limits() {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<int64_t> cache[kMaxDevices];  // (per_cu << 32 | per_block) + 1, 0 unset
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev >= 0 && dev < kMaxDevices) {
+    const int64_t v = cache[dev].load(std::memory_order_relaxed);
+    if (v) return {(int)((v - 1) >> 32), (int)((v - 1) & 0xffffffff)};
+  }
+  int per_cu = 0, per_block = 0;
+  hip_check(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev), "attr");
+  hip_check(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
+  if (dev >= 0 && dev < kMaxDevices)
+    cache[dev].store((((int64_t)per_cu << 32) | (uint32_t)per_block) + 1, std::memory_order_relaxed);
+  return {per_cu, per_block};
+}
+
 size_t staged_lds_per_wave(int prec, int u) {
   return (size_t)u * 64 * ((prec == 0 ? sizeof(double) : sizeof(float)) + sizeof(int32_t));
 }
@@ -467,11 +491,8 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
   // dynamic LDS only caps the number of resident blocks per CU
   size_t lds = 0;
   if (cfg.blocks_per_cu > 0) {
-    int dev = 0, per_cu = 0, per_block = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    hip_check(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev), "attr");
-    hip_check(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
-    lds = std::min<size_t>((size_t)per_cu / (size_t)cfg.blocks_per_cu, (size_t)per_block);
+    const LdsLimits lim = lds_limits();
+    lds = std::min<size_t>((size_t)lim.per_cu / (size_t)cfg.blocks_per_cu, (size_t)lim.per_block);
   }
 
   // k = sum of power-of-two tiles: as many k_tile tiles as fit, then the rest
@@ -494,12 +515,8 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
       const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
       const size_t need = staged_lds_per_wave(prec, cfg.unroll) * (size_t)wpb;
-      {
-        int dev = 0, per_block = 0;
-        hip_check(hipGetDevice(&dev), "hipGetDevice");
-        hip_check(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
-        if (need > (size_t)per_block) fail(WK_ERR_ARG, "LDS staging: waves_per_block * unroll too large");
-      }
+      if (need > (size_t)lds_limits().per_block)
+        fail(WK_ERR_ARG, "LDS staging: waves_per_block * unroll too large");
       for (int32_t t = 0; t < ntiles; t++) {
         a.x = static_cast<const char *>(x) + (size_t)(j0 + t) * xsz;
         a.y = y + j0 + t;
