@@ -220,6 +220,10 @@ int wk_engine_create_device(int32_t variant, int32_t threshold, int32_t max_tpr,
 #define WK_ENGINE_VALUES 16   /* double or float [allocate_nz]  device_values   hpp:782 */
 #define WK_ENGINE_COLS 17     /* int32 [allocate_nz]            device_colinds  hpp:783 */
 #define WK_ENGINE_COLS_KR 18  /* int32 [allocate_nz]            Kr columns (WK_ENGINE_KR) */
+#define WK_ENGINE_WIN_IDX 19  /* uint16 [allocate_nz]  x-window position of each slot */
+#define WK_ENGINE_WIN_OFF 20  /* uint32 [nwarps+1]     window offsets per group */
+#define WK_ENGINE_WIN_COLS 21 /* int32 [WIN_OFF[nwarps]] window columns, ascending per group */
+#define WK_ENGINE_WIN_COLS_KR 22 /* int32, the window columns in Kr numbering */
 
 /* Copies one of the engine's device arrays of the forward layout to the host.
  * host_dst NULL: *count receives the array's length (0 for an array the engine
@@ -239,9 +243,13 @@ typedef struct wk_launch {
                               by reserving LDS (occupancy control) */
   int32_t stage_lds;       /* 1: copy value/index batches through LDS with
                               16-byte loads; 2: the same with the next batch's
-                              loads issued before the current batch's FMAs.
-                              Applies to k_tile 1, group width 64, rows_per_lane
-                              1, unroll 8/16, nontemporal 1; otherwise ignored */
+                              loads issued before the current batch's FMAs;
+                              3: as 2, and x comes from an LDS window of the
+                              group's distinct columns with 16-bit positions
+                              per slot (engines whose groups have at most 2048
+                              distinct columns; otherwise as 2).  Applies to
+                              k_tile 1, group width 64, rows_per_lane 1, unroll
+                              8/16, nontemporal 1; otherwise ignored */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);

@@ -1,0 +1,153 @@
+"""x windows (stage_lds = 3) on the GPU through the C ABI.
+
+Each group's distinct columns are gathered once into LDS and every slot
+carries a 16-bit position in that list; the products and their order do not
+change, so the bars are those of the other kernels: FP64 y bit-exact against
+the oracle (CSR fma chain for WPK1, lane emulation for WPK2), run, run_x and
+the residual; FP32 engines bit-identical to the staged kernel without windows.
+Structure: every slot's window entry reproduces its padded column, in the
+plain and in the Kr numbering (pads read column 0 of either).  Layouts whose
+groups exceed the window limit, and engines whose columns were swapped, run
+the staged kernel instead."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import random_csr
+from oracle import pyoracle as O
+from warpkernel_amd import engine, fe, structure, structure2
+from warpkernel_amd import _capi as capi
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+WIN = dict(stage_lds=3, nontemporal=1, rows_per_lane=1, k_tile=1)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def nan_vec(n):
+    return torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def check_windows(eng, L, cols):
+    """Every slot's window entry is its padded column (plain and Kr)."""
+    idx = eng.download(capi.WK_ENGINE_WIN_IDX).astype(np.int64)
+    off = eng.download(capi.WK_ENGINE_WIN_OFF).astype(np.int64)
+    wc = eng.download(capi.WK_ENGINE_WIN_COLS)
+    wk = eng.download(capi.WK_ENGINE_WIN_COLS_KR)
+    assert idx.size == L.allocate_nz and off.size == L.nwarps + 1
+    group = np.repeat(np.arange(L.nwarps), L.max_nz_rows.astype(np.int64) * 64)
+    _, pc = O.process(L, np.zeros(cols.size), cols)
+    assert np.array_equal(wc[off[group] + idx], pc)
+    assert np.array_equal(wk[off[group] + idx], O.reorder_columns_coalesced(L, cols))
+    sizes = np.diff(off)
+    assert sizes.max() <= 2048
+    for g in range(0, L.nwarps, max(1, L.nwarps // 50)):  # ascending, distinct
+        w = wc[off[g]:off[g + 1]]
+        assert np.all(np.diff(w) >= 0)
+    return int(sizes.max())
+
+
+@pytest.mark.parametrize("name,th", [("hex_elastic_3d_32", 27), ("tet_poisson_3d_96", 5),
+                                      ("tet_poisson_3d_96_perm", 5), ("q1_laplace_2d_128", 4)])
+@pytest.mark.parametrize("u,wpb", [(8, 4), (16, 1), (8, 2)])
+def test_fe_window_run_run_x_residual(name, th, u, wpb):
+    say(f"[gpu] window {name} u={u} wpb={wpb}")
+    m = fe.CONFIGS[name]()
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-1, 1, m.ncols)
+    bvec = rng.uniform(-1, 1, m.nrows)
+    cfg = dict(WIN, unroll=u, waves_per_block=wpb)
+    for variant in (1, 2):
+        if variant == 1:
+            s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+            L = O.scan1(m.rowptr, 64)
+            ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+        else:
+            s = structure2(64, 32).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, th)
+            L = O.scan2(m.rowptr, th, 64, 32)
+            pv, pc = O.process(L, m.vals, m.colind)
+            ref = O.wpk_spmv(L, pv, pc, x)
+        eng = engine(s, m.vals, m.colind, kr=True)
+        if u == 8 and wpb == 4:
+            check_windows(eng, L, m.colind)
+        y = nan_vec(m.nrows)
+        eng.launch(dev(x), y, cfg=cfg)
+        assert np.array_equal(host(y), ref)
+        yx = nan_vec(m.nrows)
+        eng.launch_x(dev(O.reorder_x(L, x)), yx, cfg=cfg)
+        assert np.array_equal(host(yx), ref[L.row_map])
+        r = nan_vec(m.nrows)
+        eng.residual(dev(bvec), dev(x), r, cfg=cfg)
+        assert np.array_equal(host(r), bvec - ref)
+        eng.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_random_window_or_fallback(mode):
+    """Row-length mixes up to 5000 entries: some layouts exceed the window limit
+    and run the staged kernel; all are bit-exact."""
+    say(f"[gpu] window random mode={mode}")
+    rng = np.random.default_rng(300 + mode)
+    n = 6000
+    rp, cols, vals = random_csr(rng, n, n, mode)
+    x = rng.uniform(-1, 1, n)
+    ref = O.csr_spmv(rp, cols, vals, x)
+    s = structure(64).scan(int(rp[-1]), n, vals, rp, cols)
+    eng = engine(s, vals, cols, kr=True)
+    y = nan_vec(n)
+    eng.launch(dev(x), y, cfg=dict(WIN, unroll=8, waves_per_block=4))
+    assert np.array_equal(host(y), ref)
+    if eng.download(capi.WK_ENGINE_WIN_OFF).size:
+        check_windows(eng, O.scan1(rp, 64), cols)
+    eng.close()
+
+
+def test_fp32_window_matches_staged():
+    say("[gpu] window fp32")
+    m = fe.CONFIGS["hex_elastic_3d_32"]()
+    s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind, dtype="f32")
+    x = dev(fe.uniform_x(m.ncols, seed=5).astype(np.float32))
+    for acc in (0, 1):
+        y2, y3 = nan_vec(m.nrows), nan_vec(m.nrows)
+        eng.launch(x, y2, cfg=dict(WIN, stage_lds=2, unroll=8, waves_per_block=4, accumulate=acc))
+        eng.launch(x, y3, cfg=dict(WIN, unroll=8, waves_per_block=4, accumulate=acc))
+        assert torch.equal(y2, y3)
+    eng.close()
+
+
+def test_swapped_columns_drop_the_windows():
+    """The drivers' device_colinds swap (wpk1.cu:175) replaces the columns the
+    windows were built from: launches run the staged kernel from then on."""
+    say("[gpu] window after device_colinds swap")
+    m = fe.CONFIGS["q1_laplace_2d_128"]()
+    x = fe.uniform_x(m.ncols, seed=9)
+    ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind)
+    restore = eng.device_colinds
+    eng.device_colinds = s.reorder_columns_coalesced(m.colind)
+    yx = nan_vec(m.nrows)
+    eng.launch_x(dev(s.reorder_x(x)), yx, cfg=dict(WIN, unroll=8, waves_per_block=4))
+    assert np.array_equal(host(yx), ref[s.row_map])
+    eng.device_colinds = restore
+    y = nan_vec(m.nrows)
+    eng.launch(dev(x), y, cfg=dict(WIN, unroll=8, waves_per_block=4))
+    assert np.array_equal(host(y), ref)
+    eng.close()

@@ -20,6 +20,7 @@ WK_MAX_NZ_ROWS, WK_THREADS_PER_ROW, WK_ROW_OFFSET_WARP, WK_REORDER_A_ROWS = 4, 5
 WK_ROWPTR = 8
 WK_ENGINE_KR, WK_ENGINE_ADJOINT, WK_ENGINE_ROWSORT = 1, 2, 4
 WK_ENGINE_VALUES, WK_ENGINE_COLS, WK_ENGINE_COLS_KR = 16, 17, 18
+WK_ENGINE_WIN_IDX, WK_ENGINE_WIN_OFF, WK_ENGINE_WIN_COLS, WK_ENGINE_WIN_COLS_KR = 19, 20, 21, 22
 
 
 class WKError(RuntimeError):

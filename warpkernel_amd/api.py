@@ -262,7 +262,9 @@ class engine:
               capi.WK_MAX_NZ_ROWS: np.uint32, capi.WK_THREADS_PER_ROW: np.uint32,
               capi.WK_ROW_OFFSET_WARP: np.uint32,
               capi.WK_ENGINE_VALUES: np.float64 if self.dtype == capi.WK_F64 else np.float32,
-              capi.WK_ENGINE_COLS: np.int32, capi.WK_ENGINE_COLS_KR: np.int32}[which]
+              capi.WK_ENGINE_COLS: np.int32, capi.WK_ENGINE_COLS_KR: np.int32,
+              capi.WK_ENGINE_WIN_IDX: np.uint16, capi.WK_ENGINE_WIN_OFF: np.uint32,
+              capi.WK_ENGINE_WIN_COLS: np.int32, capi.WK_ENGINE_WIN_COLS_KR: np.int32}[which]
         n = C.c_int64()
         check(lib().wk_engine_download(self._h, which, None, C.byref(n)))
         out = np.empty(n.value, dt)

@@ -77,6 +77,32 @@ void upload_layout(wk_engine *E, DevLayout &D, const wk_layout &L, const double 
   }
   D.slot_of_nz = dev_upload(E, slot_of_nz ? slot_of_nz : L.reorder_rows.data(), L.nz, s);
   hip_check(hipStreamSynchronize(s), "upload");
+  if (L.W == 64) {
+    // x windows: one key per slot, the entry's column or -1 for a pad
+    std::vector<int32_t> keys((size_t)L.allocate_nz, -1);
+    parallel_for(L.nz, 1 << 20, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; i++) keys[L.reorder_rows[i]] = colind[i];
+    });
+    std::vector<uint16_t> idx;
+    std::vector<uint32_t> off;
+    std::vector<int32_t> wkeys;
+    if (build_windows(L.warp_offsets, L.max_nz_rows, L.W, keys.data(), idx, off, wkeys, D.win_max)) {
+      std::vector<int32_t>().swap(keys);
+      std::vector<int32_t> wc(wkeys.size());
+      for (size_t j = 0; j < wkeys.size(); j++) wc[j] = wkeys[j] < 0 ? 0 : wkeys[j];
+      D.win_idx = dev_upload(E, idx.data(), (int64_t)idx.size(), s);
+      D.win_off = dev_upload(E, off.data(), (int64_t)off.size(), s);
+      D.win_cols = dev_upload(E, wc.data(), (int64_t)wc.size(), s);
+      hip_check(hipStreamSynchronize(s), "upload");
+      if (kr) {
+        for (size_t j = 0; j < wkeys.size(); j++) wc[j] = wkeys[j] < 0 ? 0 : L.row_map_inv[wkeys[j]];
+        D.win_cols_kr = dev_upload(E, wc.data(), (int64_t)wc.size(), s);
+        hip_check(hipStreamSynchronize(s), "upload");
+      }
+    } else {
+      D.win_max = 0;
+    }
+  }
 }
 
 wk_launch resolve(wk_engine *E, int32_t op, int32_t k, const wk_launch *cfg) {
@@ -348,6 +374,9 @@ int wk_engine_set_colinds(wk_engine *E, const int32_t *pcols, int32_t where, voi
     hip_check(hipMemcpyAsync(E->fwd.cols, pcols, bytes,
                              where ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s),
               "set_colinds");
+    // the x windows describe the old columns: launches fall back to staging
+    // without windows from here on
+    E->fwd.win_max = 0;
     if (!where) hip_check(hipStreamSynchronize(s), "set_colinds");
   });
 }

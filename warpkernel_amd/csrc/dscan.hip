@@ -435,6 +435,17 @@ extern "C" int wk_engine_download(const wk_engine *E, int32_t which, void *host_
         break;
       case WK_ENGINE_COLS: src = D.cols; n = D.cols ? D.allocate_nz : 0; break;
       case WK_ENGINE_COLS_KR: src = D.cols_kr; n = D.cols_kr ? D.allocate_nz : 0; break;
+      case WK_ENGINE_WIN_IDX: src = D.win_idx; n = D.win_idx ? D.allocate_nz : 0; esz = 2; break;
+      case WK_ENGINE_WIN_OFF: src = D.win_off; n = D.win_off ? D.ngroups + 1 : 0; break;
+      case WK_ENGINE_WIN_COLS:
+      case WK_ENGINE_WIN_COLS_KR: {
+        src = which == WK_ENGINE_WIN_COLS ? D.win_cols : D.win_cols_kr;
+        uint32_t total = 0;
+        if (src && D.win_off)
+          hip_check(hipMemcpy(&total, D.win_off + D.ngroups, sizeof total, hipMemcpyDeviceToHost), "download");
+        n = src ? total : 0;
+        break;
+      }
       default: fail(WK_ERR_ARG, "unknown engine array");
     }
     if (!count) fail(WK_ERR_ARG, "count is NULL");

@@ -229,6 +229,67 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_staged_kernel(SpmvArgs a, in
   staged_block<P, SPLIT, OUT, U, PIPE>(a, (int32_t)blockIdx.x, nwork, remap, lds);
 }
 
+// ------------------------------------------------------- x-window kernel
+// Same result as spmv_staged_kernel; each wave first gathers x at its group's
+// distinct columns into its own LDS window (the group's columns come from a
+// few L2 gathers instead of one per slot), then streams values and 16-bit
+// window positions (10 bytes per slot instead of 12) and reads x from LDS.
+struct WinArgs {
+  const uint16_t *idx;   // [allocate_nz] window position of each slot
+  const uint32_t *off;   // [ngroups+1]
+  const int32_t *cols;   // window columns in the product's numbering
+  int32_t wave_bytes;    // LDS per wave: window, then slice
+  int32_t win_bytes;     // window part (16-byte multiple)
+};
+
+template <class P, bool SPLIT, int OUT, int U>
+__global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
+                                                               int32_t remap) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  constexpr int W = 64;
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t wpb = blockDim.x >> 6;
+  const int32_t bid = blockIdx.x;
+  if (bid >= nwork) {
+    const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
+    if (pos >= a.nrows) return;
+    const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
+    a.y[dst * a.ldy] = OUT == OUT_RESID ? a.b[dst * a.ldb] : 0.0;
+    return;
+  }
+  const int32_t wib = threadIdx.x >> 6;
+  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(bid, nwork, remap) * wpb + wib);
+  if (wave >= a.ngroups) return;
+  const GroupMeta m = load_meta<W, SPLIT>(a, wave);
+  const V *vbase = static_cast<const V *>(a.vals) + m.base;
+  const uint16_t *ibase = w.idx + m.base;
+  char *mine = lds + wib * w.wave_bytes;
+  X *xs = reinterpret_cast<X *>(mine);
+  WinBatch<P, U> bt;
+  const bool early = m.len >= (uint32_t)U;
+  if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
+  const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
+  fill_window<X>(xs, w.cols + w0, w1 - w0, static_cast<const X *>(a.x), a.ldx, lane);
+  Acc acc = window_group<P, U>(vbase, ibase, xs, m.len, bt, early, mine + w.win_bytes, lane);
+  if constexpr (SPLIT) {
+    for (uint32_t o = 1; o < m.tpr; o <<= 1) {
+      Acc other = P::shfl_down(acc, (int)o, W);
+      P::add(acc, other);
+    }
+  }
+  const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
+  if (((uint32_t)lane & (m.tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
+  const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
+  const double res = P::result(acc);
+  if constexpr (OUT == OUT_RESID)
+    a.y[dst * a.ldy] = a.b[dst * a.ldb] - res;
+  else
+    a.y[dst * a.ldy] = res;
+}
+
 // ------------------------------------------------------------ small kernels
 __global__ void fill_zero_kernel(uint32_t *p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -447,6 +508,41 @@ LdsLimits lds_limits() {
   return {per_cu, per_block};
 }
 
+using WinFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
+                       hipStream_t);
+
+template <class P, bool SPLIT, int OUT, int U>
+void launch_win(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
+                int32_t nwork, int32_t remap, hipStream_t s) {
+  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U>), grid, block, lds, s, a, w, nwork, remap);
+}
+
+template <class P, bool SPLIT, int OUT>
+WinFn pick_win_u(int u) {
+  if (u == 8) return &launch_win<P, SPLIT, OUT, 8>;
+  if (u == 16) return &launch_win<P, SPLIT, OUT, 16>;
+  return nullptr;
+}
+
+template <class P, bool SPLIT>
+WinFn pick_win_out(int out, int u) {
+  if (out == OUT_RUN) return pick_win_u<P, SPLIT, OUT_RUN>(u);
+  if (out == OUT_RUNX) return pick_win_u<P, SPLIT, OUT_RUNX>(u);
+  if constexpr (std::is_same_v<P, PF64>) return pick_win_u<P, SPLIT, OUT_RESID>(u);
+  return nullptr;
+}
+
+template <class P>
+WinFn pick_win_split(bool split, int out, int u) {
+  return split ? pick_win_out<P, true>(out, u) : pick_win_out<P, false>(out, u);
+}
+
+WinFn pick_win(int prec, bool split, int out, int u) {
+  if (prec == 0) return pick_win_split<PF64>(split, out, u);
+  if (prec == 1) return pick_win_split<PF32C>(split, out, u);
+  return pick_win_split<PF32D>(split, out, u);
+}
+
 size_t staged_lds_per_wave(int prec, int u) {
   return (size_t)u * 64 * ((prec == 0 ? sizeof(double) : sizeof(float)) + sizeof(int32_t));
 }
@@ -463,6 +559,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
                  int64_t ldx, double *y, int64_t ldy, const double *b, int64_t ldb, int32_t k,
                  hipStream_t stream) {
   if (k < 1) fail(WK_ERR_ARG, "k must be >= 1");
+  if (cfg.stage_lds < 0 || cfg.stage_lds > 3) fail(WK_ERR_ARG, "stage_lds must be 0, 1, 2 or 3");
   const int wpb = cfg.waves_per_block;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) fail(WK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
   if (cfg.k_tile < 1 || cfg.k_tile > 32 || (cfg.k_tile & (cfg.k_tile - 1)))
@@ -509,8 +606,41 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     // per wave, unroll 8 or 16, nontemporal values); the direct kernel otherwise
     const bool staged = cfg.stage_lds && kt == 1 && D.W == 64 && r == 1 && cfg.nontemporal &&
                         (cfg.unroll == 8 || cfg.unroll == 16);
+    if (staged && cfg.stage_lds == 3 && D.win_max > 0) {
+      // x windows: per wave the window (16-byte multiple) and a U-step slice
+      const size_t xbytes = prec == 0 ? sizeof(double) : sizeof(float);
+      const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
+      const size_t win_bytes = ((size_t)D.win_max * xbytes + 15) / 16 * 16;
+      const size_t wave_bytes = win_bytes + (size_t)cfg.unroll * 64 * (vbytes + sizeof(uint16_t));
+      const size_t need = wave_bytes * (size_t)wpb;
+      WinFn wf = pick_win(prec, split, out, cfg.unroll);
+      if (wf && need <= (size_t)lds_limits().per_block) {
+        WinArgs w{};
+        w.idx = D.win_idx;
+        w.off = D.win_off;
+        w.cols = (out == OUT_RUNX && D.win_cols_kr) ? D.win_cols_kr : D.win_cols;
+        w.wave_bytes = (int32_t)wave_bytes;
+        w.win_bytes = (int32_t)win_bytes;
+        const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
+        const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+        if (nb + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+        for (int32_t t = 0; t < ntiles; t++) {
+          a.x = static_cast<const char *>(x) + (size_t)(j0 + t) * xsz;
+          a.y = y + j0 + t;
+          a.b = b ? b + j0 + t : nullptr;
+          a.k = 1;
+          if (nb + nf > 0)
+            wf(a, w, dim3((uint32_t)(nb + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nb,
+               cfg.xcd_remap ? 1 : 0, stream);
+          hip_check(hipGetLastError(), "spmv launch");
+        }
+        j0 += ntiles;
+        continue;
+      }
+      // the window does not fit this block shape: staging without windows
+    }
     if (staged) {
-      StagedFn sf = pick_staged(prec, split, out, cfg.unroll, cfg.stage_lds == 2);
+      StagedFn sf = pick_staged(prec, split, out, cfg.unroll, cfg.stage_lds >= 2);
       if (!sf) fail(WK_ERR_ARG, "unsupported staged configuration");
       const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
       const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);

@@ -286,6 +286,50 @@ void reorder_columns_rowsort(const wk_layout &L, const int32_t *rowptr, const in
   });
 }
 
+bool build_windows(const std::vector<int64_t> &warp_offsets, const std::vector<uint32_t> &max_nz_rows,
+                   int32_t W, const int32_t *keys, std::vector<uint16_t> &idx,
+                   std::vector<uint32_t> &off, std::vector<int32_t> &cols, int32_t &win_max) {
+  const int32_t G = (int32_t)max_nz_rows.size();
+  idx.clear();
+  off.clear();
+  cols.clear();
+  win_max = 0;
+  if (!G) return false;
+  // pass 1: distinct columns per group (sorted), in per-thread chunks
+  std::vector<std::vector<int32_t>> uniq((size_t)G);
+  std::atomic<bool> too_big{false};
+  parallel_for(G, 64, [&](int64_t g0, int64_t g1) {
+    for (int64_t g = g0; g < g1 && !too_big.load(std::memory_order_relaxed); g++) {
+      const int64_t a = warp_offsets[g], b = a + (int64_t)max_nz_rows[g] * W;
+      std::vector<int32_t> &u = uniq[g];
+      u.assign(keys + a, keys + b);
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      if ((int64_t)u.size() > kMaxWindow) too_big.store(true);
+    }
+  });
+  if (too_big.load()) return false;
+  off.assign((size_t)G + 1, 0);
+  for (int32_t g = 0; g < G; g++) {
+    off[g + 1] = off[g] + (uint32_t)uniq[g].size();
+    win_max = std::max<int32_t>(win_max, (int32_t)uniq[g].size());
+  }
+  cols.resize(off[G]);
+  idx.resize((size_t)warp_offsets[G]);
+  // pass 2: each slot's position in its group's list
+  parallel_for(G, 64, [&](int64_t g0, int64_t g1) {
+    for (int64_t g = g0; g < g1; g++) {
+      const std::vector<int32_t> &u = uniq[g];
+      std::copy(u.begin(), u.end(), cols.begin() + off[g]);
+      const int64_t a = warp_offsets[g], b = a + (int64_t)max_nz_rows[g] * W;
+      for (int64_t s = a; s < b; s++)
+        idx[s] = (uint16_t)(std::lower_bound(u.begin(), u.end(), keys[s]) - u.begin());
+      std::vector<int32_t>().swap(uniq[g]);
+    }
+  });
+  return true;
+}
+
 void reorder_columns_coalesced(const wk_layout &L, const int32_t *colind, int32_t *out) {
   parallel_for(L.allocate_nz, 1 << 20, [&](int64_t a, int64_t b) {
     std::memset(out + a, 0, sizeof(int32_t) * (size_t)(b - a));

@@ -279,5 +279,121 @@ __device__ __forceinline__ typename P::Acc staged_group(const typename P::V *vba
   return acc;
 }
 
+// Bytes of one wave's LDS for the window kernel: the x window (win_max
+// entries, rounded up to 16 bytes), then a U-step slice of values and 16-bit
+// window positions.
+template <class P, int U>
+__host__ __device__ constexpr int win_slice_bytes() {
+  return U * 64 * (int)(sizeof(typename P::V) + sizeof(uint16_t));
+}
+
+// One group through its x window.  xs holds x at the group's distinct columns
+// (filled by fill_window); each slot carries the 16-bit position of its column
+// there.  Values and positions stream in U-step batches with 16-byte loads
+// through the slice, the next batch in flight during the current FMAs, and x
+// comes from LDS: the same products in the same order as staged_group, without
+// a column index or an x gather per slot.  The caller has issued the first
+// batch (tv/ti) before filling the window.
+template <class P, int U>
+struct WinBatch {
+  static constexpr int NV = U * 64 * (int)sizeof(typename P::V) / 1024;
+  static constexpr int NI = U * 64 * 2 / 1024;
+  static_assert(NV >= 1 && NI >= 1, "a batch covers whole 1 KB chunks");
+  u32x4 tv[NV], ti[NI];
+  __device__ __forceinline__ void issue(const typename P::V *vbase, const uint16_t *ibase,
+                                        uint32_t at, int32_t lane) {
+    const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)at * 64);
+    const u32x4 *gi = reinterpret_cast<const u32x4 *>(ibase + (int64_t)at * 64);
+#pragma unroll
+    for (int q = 0; q < NI; q++) ti[q] = gi[q * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+  }
+  __device__ __forceinline__ void stage(char *slice, int32_t lane) const {
+    u32x4 *lv = reinterpret_cast<u32x4 *>(slice);
+    u32x4 *li = reinterpret_cast<u32x4 *>(slice + U * 64 * sizeof(typename P::V));
+#pragma unroll
+    for (int q = 0; q < NI; q++) li[q * 64 + lane] = ti[q];
+#pragma unroll
+    for (int q = 0; q < NV; q++) lv[q * 64 + lane] = tv[q];
+  }
+};
+
+// Fills the window: xs[j] = x[wcols[j] * ldx], j < n, 8 entries per lane in
+// flight at a time.
+template <class X>
+__device__ __forceinline__ void fill_window(X *xs, const int32_t *__restrict__ wcols, uint32_t n,
+                                            const X *__restrict__ xb, int64_t ldx, int32_t lane) {
+  constexpr int C = 8;
+  for (uint32_t j0 = 0; j0 < n; j0 += C * 64) {
+    int32_t c[C];
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const uint32_t j = j0 + q * 64 + lane;
+      c[q] = j < n ? wcols[j] : 0;
+    }
+    X v[C];
+#pragma unroll
+    for (int q = 0; q < C; q++) v[q] = xb[(int64_t)c[q] * ldx];
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const uint32_t j = j0 + q * 64 + lane;
+      if (j < n) xs[j] = v[q];
+    }
+  }
+}
+
+template <class P, int U>
+__device__ __forceinline__ typename P::Acc window_group(const typename P::V *vbase,
+                                                        const uint16_t *ibase, const typename P::X *xs,
+                                                        uint32_t len, WinBatch<P, U> &bt, bool issued,
+                                                        char *slice, int32_t lane) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  const V *lv = reinterpret_cast<const V *>(slice);
+  const uint16_t *li = reinterpret_cast<const uint16_t *>(slice + U * 64 * sizeof(V));
+  Acc acc;
+  P::init(acc);  // every slot by fma onto -0
+  uint32_t i = 0;
+  if (i + U <= len) {
+    if (!issued) bt.issue(vbase, ibase, 0, lane);
+    for (; i + 2 * U <= len; i += U) {
+      bt.stage(slice, lane);
+      X xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) xv[u] = xs[li[u * 64 + lane]];
+      bt.issue(vbase, ibase, i + U, lane);
+      V v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) v[u] = lv[u * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < U; u++) P::step(acc, v[u], xv[u]);
+    }
+    bt.stage(slice, lane);
+    X xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = xs[li[u * 64 + lane]];
+#pragma unroll
+    for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+    i += U;
+  }
+  // the last, partial batch: direct loads, all issued before the first use
+  if (i < len) {
+    V v[U];
+    uint16_t c[U];
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) {
+        c[u] = ibase[(int64_t)(i + u) * 64 + lane];
+        v[u] = __builtin_nontemporal_load(vbase + (int64_t)(i + u) * 64 + lane);
+      }
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) P::step(acc, v[u], xs[c[u]]);
+  }
+  return acc;
+}
+
 }  // namespace
 }  // namespace wk

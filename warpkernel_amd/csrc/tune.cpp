@@ -47,6 +47,11 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
               c.stage_lds = st;
               out.push_back(c);
             }
+            // x windows (the launcher falls back to 2 where a window does not fit)
+            if (D.win_max > 0) {
+              c.stage_lds = 3;
+              out.push_back(c);
+            }
             c.stage_lds = 0;
           }
         }
@@ -75,9 +80,17 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t k) {
   wk_launch_default(&c);
   const double mean_len =
       D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * D.W) : 0.0;
-  if (k == 1 && D.W == 64 && mean_len >= 12.0) {
+  if (k == 1 && D.W == 64 && mean_len >= 40.0 && D.win_max > 0) {
+    // long groups: x windows in LDS, 10 bytes per slot (measured best on hex
+    // elasticity, 81-entry rows: 11% over staging without windows); the
+    // window fill costs too much per group for short rows
+    c.stage_lds = 3;
+    c.unroll = 8;
+    c.waves_per_block = 2;
+    c.xcd_remap = 0;
+  } else if (k == 1 && D.W == 64 && mean_len >= 12.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
-    // (measured best on hex elasticity and hex diffusion)
+    // (measured best on hex diffusion and P1 tets)
     c.stage_lds = 2;
     c.unroll = 8;
     c.waves_per_block = 4;

@@ -85,7 +85,27 @@ struct DevLayout {
   uint32_t *row_off = nullptr;       // [ngroups+1] (variant 2)
   uint32_t *slot_of_nz = nullptr;    // [nnz] CSR index -> padded slot (GPU process)
   std::vector<uint32_t> h_max_nz_rows;  // host copy (batch work partition)
+  // x windows (stage_lds 3): each group's distinct columns, ascending, and per
+  // slot the 16-bit position of its column in that list.  win_max = 0: none.
+  uint16_t *win_idx = nullptr;     // [allocate_nz]
+  uint32_t *win_off = nullptr;     // [ngroups+1] into win_cols
+  int32_t *win_cols = nullptr;     // [win_off[ngroups]]
+  int32_t *win_cols_kr = nullptr;  // the same columns in Kr numbering (row_map_inv)
+  int32_t win_max = 0;             // largest window; 0 when the layout has none
 };
+
+// Largest x window (distinct columns of one group) that the window kernel
+// stages; layouts with a larger group get no windows.
+constexpr int32_t kMaxWindow = 2048;
+
+// Builds the x windows of a layout from one key per padded slot: the column of
+// an entry, -1 for a pad (a pad reads column 0 of whichever numbering the
+// product uses, which in the Kr form differs from an entry of column 0).
+// keys_out lists each group's distinct keys ascending.  Returns false, leaving
+// the outputs empty, when a group has more than kMaxWindow distinct keys.
+bool build_windows(const std::vector<int64_t> &warp_offsets, const std::vector<uint32_t> &max_nz_rows,
+                   int32_t W, const int32_t *keys, std::vector<uint16_t> &idx,
+                   std::vector<uint32_t> &off, std::vector<int32_t> &keys_out, int32_t &win_max);
 
 }  // namespace wk
 
