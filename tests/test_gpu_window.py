@@ -151,3 +151,52 @@ def test_swapped_columns_drop_the_windows():
     eng.launch(dev(x), y, cfg=dict(WIN, unroll=8, waves_per_block=4))
     assert np.array_equal(host(y), ref)
     eng.close()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 8, 16, 32])
+def test_multi_rhs_all_paths_bitexact(k):
+    """k right-hand sides through the default choice, the direct kernel at every
+    k-tile and the windowed kernel at k-tiles 2, 4 and 8: FP64 bit-exact against
+    the oracle (WPK1 CSR order, WPK2 lane order), run and run_x."""
+    say(f"[gpu] multi-RHS k={k}")
+    m = fe.CONFIGS["hex_elastic_3d_32"]() if k in (4, 8) else fe.tet_poisson_3d(40, drop=0.05)
+    x = fe.uniform_x(m.ncols, k, seed=4000 + k).reshape(m.ncols, k)
+    cfgs = [None] + [dict(k_tile=t, stage_lds=0) for t in (1, 2, 4, 8, 16, 32) if t <= k] + \
+        [dict(WIN, k_tile=t, unroll=8, waves_per_block=w) for t in (2, 4, 8) if t <= k for w in (1, 2)]
+    for variant in (1, 2):
+        if variant == 1:
+            s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+            L = O.scan1(m.rowptr, 64)
+            ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+        else:
+            s = structure2(64, 32).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, 5)
+            L = O.scan2(m.rowptr, 5, 64, 32)
+            pv, pc = O.process(L, m.vals, m.colind)
+            ref = O.wpk_spmv(L, pv, pc, x)
+        eng = engine(s, m.vals, m.colind, kr=True)
+        xr = np.empty_like(x)
+        xr[L.row_map_inv] = x  # reorder_x, row by row
+        xd, xrd = dev(x), dev(xr)
+        for cfg in cfgs:
+            y = torch.full((m.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
+            eng.launch(xd, y, cfg=cfg)
+            assert np.array_equal(host(y), ref), (variant, cfg)
+            eng.launch_x(xrd, y, cfg=cfg)
+            assert np.array_equal(host(y), ref[L.row_map]), (variant, cfg, "run_x")
+        eng.close()
+
+
+@pytest.mark.parametrize("k", [2, 4])
+def test_multi_rhs_fp32_windowed_matches_direct(k):
+    say(f"[gpu] multi-RHS fp32 k={k}")
+    m = fe.CONFIGS["hex_elastic_3d_32"]()
+    s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind, dtype="f32")
+    x = dev(fe.uniform_x(m.ncols, k, seed=4100 + k).reshape(m.ncols, k).astype(np.float32))
+    for acc in (0, 1):
+        y0 = torch.full((m.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
+        y1 = torch.full_like(y0, float("nan"))
+        eng.launch(x, y0, cfg=dict(k_tile=k, stage_lds=0, accumulate=acc))
+        eng.launch(x, y1, cfg=dict(WIN, k_tile=k, unroll=8, waves_per_block=2, accumulate=acc))
+        assert torch.equal(y0, y1), acc
+    eng.close()

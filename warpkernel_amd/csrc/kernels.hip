@@ -242,7 +242,7 @@ struct WinArgs {
   int32_t win_bytes;     // window part (16-byte multiple)
 };
 
-template <class P, bool SPLIT, int OUT, int U>
+template <class P, bool SPLIT, int OUT, int U, int KT>
 __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
                                                                int32_t remap) {
   using V = typename P::V;
@@ -257,7 +257,9 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
     const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
     if (pos >= a.nrows) return;
     const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
-    a.y[dst * a.ldy] = OUT == OUT_RESID ? a.b[dst * a.ldb] : 0.0;
+#pragma unroll
+    for (int t = 0; t < KT; t++)
+      a.y[dst * a.ldy + t] = OUT == OUT_RESID ? a.b[dst * a.ldb + t] : 0.0;
     return;
   }
   const int32_t wib = threadIdx.x >> 6;
@@ -272,22 +274,29 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   const bool early = m.len >= (uint32_t)U;
   if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
   const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
-  fill_window<X>(xs, w.cols + w0, w1 - w0, static_cast<const X *>(a.x), a.ldx, lane);
-  Acc acc = window_group<P, U>(vbase, ibase, xs, m.len, bt, early, mine + w.win_bytes, lane);
+  fill_window<X, KT>(xs, w.cols + w0, w1 - w0, static_cast<const X *>(a.x), a.ldx, lane);
+  Acc acc[KT];
+  window_group<P, U, KT>(acc, vbase, ibase, xs, m.len, bt, early, mine + w.win_bytes, lane);
   if constexpr (SPLIT) {
     for (uint32_t o = 1; o < m.tpr; o <<= 1) {
-      Acc other = P::shfl_down(acc, (int)o, W);
-      P::add(acc, other);
+#pragma unroll
+      for (int t = 0; t < KT; t++) {
+        Acc other = P::shfl_down(acc[t], (int)o, W);
+        P::add(acc[t], other);
+      }
     }
   }
   const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
   if (((uint32_t)lane & (m.tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
   const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
-  const double res = P::result(acc);
-  if constexpr (OUT == OUT_RESID)
-    a.y[dst * a.ldy] = a.b[dst * a.ldb] - res;
-  else
-    a.y[dst * a.ldy] = res;
+#pragma unroll
+  for (int t = 0; t < KT; t++) {
+    const double res = P::result(acc[t]);
+    if constexpr (OUT == OUT_RESID)
+      a.y[dst * a.ldy + t] = a.b[dst * a.ldb + t] - res;
+    else
+      a.y[dst * a.ldy + t] = res;
+  }
 }
 
 // ------------------------------------------------------------ small kernels
@@ -511,36 +520,45 @@ LdsLimits lds_limits() {
 using WinFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
                        hipStream_t);
 
-template <class P, bool SPLIT, int OUT, int U>
+template <class P, bool SPLIT, int OUT, int U, int KT>
 void launch_win(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
                 int32_t nwork, int32_t remap, hipStream_t s) {
-  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U>), grid, block, lds, s, a, w, nwork, remap);
+  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U, KT>), grid, block, lds, s, a, w, nwork,
+                     remap);
 }
 
+// k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8
 template <class P, bool SPLIT, int OUT>
-WinFn pick_win_u(int u) {
-  if (u == 8) return &launch_win<P, SPLIT, OUT, 8>;
-  if (u == 16) return &launch_win<P, SPLIT, OUT, 16>;
+WinFn pick_win_u(int u, int kt) {
+  if (kt == 1) {
+    if (u == 8) return &launch_win<P, SPLIT, OUT, 8, 1>;
+    if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1>;
+    return nullptr;
+  }
+  if (u != 8) return nullptr;
+  if (kt == 2) return &launch_win<P, SPLIT, OUT, 8, 2>;
+  if (kt == 4) return &launch_win<P, SPLIT, OUT, 8, 4>;
+  if (kt == 8) return &launch_win<P, SPLIT, OUT, 8, 8>;
   return nullptr;
 }
 
 template <class P, bool SPLIT>
-WinFn pick_win_out(int out, int u) {
-  if (out == OUT_RUN) return pick_win_u<P, SPLIT, OUT_RUN>(u);
-  if (out == OUT_RUNX) return pick_win_u<P, SPLIT, OUT_RUNX>(u);
-  if constexpr (std::is_same_v<P, PF64>) return pick_win_u<P, SPLIT, OUT_RESID>(u);
+WinFn pick_win_out(int out, int u, int kt) {
+  if (out == OUT_RUN) return pick_win_u<P, SPLIT, OUT_RUN>(u, kt);
+  if (out == OUT_RUNX) return pick_win_u<P, SPLIT, OUT_RUNX>(u, kt);
+  if constexpr (std::is_same_v<P, PF64>) return pick_win_u<P, SPLIT, OUT_RESID>(u, kt);
   return nullptr;
 }
 
 template <class P>
-WinFn pick_win_split(bool split, int out, int u) {
-  return split ? pick_win_out<P, true>(out, u) : pick_win_out<P, false>(out, u);
+WinFn pick_win_split(bool split, int out, int u, int kt) {
+  return split ? pick_win_out<P, true>(out, u, kt) : pick_win_out<P, false>(out, u, kt);
 }
 
-WinFn pick_win(int prec, bool split, int out, int u) {
-  if (prec == 0) return pick_win_split<PF64>(split, out, u);
-  if (prec == 1) return pick_win_split<PF32C>(split, out, u);
-  return pick_win_split<PF32D>(split, out, u);
+WinFn pick_win(int prec, bool split, int out, int u, int kt) {
+  if (prec == 0) return pick_win_split<PF64>(split, out, u, kt);
+  if (prec == 1) return pick_win_split<PF32C>(split, out, u, kt);
+  return pick_win_split<PF32D>(split, out, u, kt);
 }
 
 size_t staged_lds_per_wave(int prec, int u) {
@@ -606,14 +624,19 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     // per wave, unroll 8 or 16, nontemporal values); the direct kernel otherwise
     const bool staged = cfg.stage_lds && kt == 1 && D.W == 64 && r == 1 && cfg.nontemporal &&
                         (cfg.unroll == 8 || cfg.unroll == 16);
-    if (staged && cfg.stage_lds == 3 && D.win_max > 0) {
-      // x windows: per wave the window (16-byte multiple) and a U-step slice
-      const size_t xbytes = prec == 0 ? sizeof(double) : sizeof(float);
+    // x windows: any k-tile up to 8 (unroll 8 for k-tiles above 1)
+    const bool windowed = cfg.stage_lds == 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
+                          cfg.nontemporal &&
+                          (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
+    if (windowed) {
+      // per wave: the window (win_max x kt entries, 16-byte multiple), then a
+      // U-step slice of values and 16-bit positions
+      const int u = kt == 1 ? cfg.unroll : 8;
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
-      const size_t win_bytes = ((size_t)D.win_max * xbytes + 15) / 16 * 16;
-      const size_t wave_bytes = win_bytes + (size_t)cfg.unroll * 64 * (vbytes + sizeof(uint16_t));
+      const size_t win_bytes = ((size_t)D.win_max * kt * xsz + 15) / 16 * 16;
+      const size_t wave_bytes = win_bytes + (size_t)u * 64 * (vbytes + sizeof(uint16_t));
       const size_t need = wave_bytes * (size_t)wpb;
-      WinFn wf = pick_win(prec, split, out, cfg.unroll);
+      WinFn wf = pick_win(prec, split, out, u, kt);
       if (wf && need <= (size_t)lds_limits().per_block) {
         WinArgs w{};
         w.idx = D.win_idx;
@@ -625,19 +648,20 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
         if (nb + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
         for (int32_t t = 0; t < ntiles; t++) {
-          a.x = static_cast<const char *>(x) + (size_t)(j0 + t) * xsz;
-          a.y = y + j0 + t;
-          a.b = b ? b + j0 + t : nullptr;
-          a.k = 1;
+          const int32_t jt = j0 + t * kt;
+          a.x = static_cast<const char *>(x) + (size_t)jt * xsz;
+          a.y = y + jt;
+          a.b = b ? b + jt : nullptr;
+          a.k = kt;
           if (nb + nf > 0)
             wf(a, w, dim3((uint32_t)(nb + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nb,
                cfg.xcd_remap ? 1 : 0, stream);
           hip_check(hipGetLastError(), "spmv launch");
         }
-        j0 += ntiles;
+        j0 += ntiles * kt;
         continue;
       }
-      // the window does not fit this block shape: staging without windows
+      // the window does not fit this block shape: the kernels without windows
     }
     if (staged) {
       StagedFn sf = pick_staged(prec, split, out, cfg.unroll, cfg.stage_lds >= 2);

@@ -319,12 +319,12 @@ struct WinBatch {
   }
 };
 
-// Fills the window: xs[j] = x[wcols[j] * ldx], j < n, 8 entries per lane in
-// flight at a time.
-template <class X>
+// Fills the window: xs[j*KT + t] = x[wcols[j] * ldx + t], j < n, t < KT,
+// 8 entries per lane in flight at a time.
+template <class X, int KT>
 __device__ __forceinline__ void fill_window(X *xs, const int32_t *__restrict__ wcols, uint32_t n,
                                             const X *__restrict__ xb, int64_t ldx, int32_t lane) {
-  constexpr int C = 8;
+  constexpr int C = KT >= 4 ? 4 : 8;
   for (uint32_t j0 = 0; j0 < n; j0 += C * 64) {
     int32_t c[C];
 #pragma unroll
@@ -332,50 +332,56 @@ __device__ __forceinline__ void fill_window(X *xs, const int32_t *__restrict__ w
       const uint32_t j = j0 + q * 64 + lane;
       c[q] = j < n ? wcols[j] : 0;
     }
-    X v[C];
+    X v[C][KT];
 #pragma unroll
-    for (int q = 0; q < C; q++) v[q] = xb[(int64_t)c[q] * ldx];
+    for (int q = 0; q < C; q++)
+#pragma unroll
+      for (int t = 0; t < KT; t++) v[q][t] = xb[(int64_t)c[q] * ldx + t];
 #pragma unroll
     for (int q = 0; q < C; q++) {
       const uint32_t j = j0 + q * 64 + lane;
-      if (j < n) xs[j] = v[q];
+      if (j < n)
+#pragma unroll
+        for (int t = 0; t < KT; t++) xs[(int64_t)j * KT + t] = v[q][t];
     }
   }
 }
 
-template <class P, int U>
-__device__ __forceinline__ typename P::Acc window_group(const typename P::V *vbase,
-                                                        const uint16_t *ibase, const typename P::X *xs,
-                                                        uint32_t len, WinBatch<P, U> &bt, bool issued,
-                                                        char *slice, int32_t lane) {
+template <class P, int U, int KT>
+__device__ __forceinline__ void window_group(typename P::Acc (&acc)[KT], const typename P::V *vbase,
+                                             const uint16_t *ibase, const typename P::X *xs,
+                                             uint32_t len, WinBatch<P, U> &bt, bool issued,
+                                             char *slice, int32_t lane) {
   using V = typename P::V;
-  using X = typename P::X;
-  using Acc = typename P::Acc;
   const V *lv = reinterpret_cast<const V *>(slice);
   const uint16_t *li = reinterpret_cast<const uint16_t *>(slice + U * 64 * sizeof(V));
-  Acc acc;
-  P::init(acc);  // every slot by fma onto -0
+#pragma unroll
+  for (int t = 0; t < KT; t++) P::init(acc[t]);  // every slot by fma onto -0
   uint32_t i = 0;
   if (i + U <= len) {
     if (!issued) bt.issue(vbase, ibase, 0, lane);
     for (; i + 2 * U <= len; i += U) {
       bt.stage(slice, lane);
-      X xv[U];
+      uint32_t p[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) xv[u] = xs[li[u * 64 + lane]];
+      for (int u = 0; u < U; u++) p[u] = (uint32_t)li[u * 64 + lane] * KT;
       bt.issue(vbase, ibase, i + U, lane);
       V v[U];
 #pragma unroll
       for (int u = 0; u < U; u++) v[u] = lv[u * 64 + lane];
 #pragma unroll
-      for (int u = 0; u < U; u++) P::step(acc, v[u], xv[u]);
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xs[p[u] + t]);
     }
     bt.stage(slice, lane);
-    X xv[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) xv[u] = xs[li[u * 64 + lane]];
+    for (int u = 0; u < U; u++) {
+      const uint32_t p = (uint32_t)li[u * 64 + lane] * KT;
+      const V v = lv[u * 64 + lane];
 #pragma unroll
-    for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+      for (int t = 0; t < KT; t++) P::step(acc[t], v, xs[p + t]);
+    }
     i += U;
   }
   // the last, partial batch: direct loads, all issued before the first use
@@ -390,9 +396,10 @@ __device__ __forceinline__ typename P::Acc window_group(const typename P::V *vba
       }
 #pragma unroll
     for (int u = 0; u < U - 1; u++)
-      if (i + u < len) P::step(acc, v[u], xs[c[u]]);
+      if (i + u < len)
+#pragma unroll
+        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xs[(uint32_t)c[u] * KT + t]);
   }
-  return acc;
 }
 
 }  // namespace
