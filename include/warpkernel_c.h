@@ -339,8 +339,9 @@ typedef struct wk_batch wk_batch;
  * dispatcher balance them; > 0 (at most 1024) runs that many persistent waves
  * per compute unit, each over a contiguous range of equal cost.  stage_lds: as
  * in wk_launch, for batches with one right-hand side (0 direct loads, 1
- * staged, 2 staged with the next batch in flight); applied to the matrices
- * whose mean group length is at least 24, the others use direct loads. */
+ * staged, 2 staged with the next batch in flight, 3 as 2 plus x windows);
+ * staging applies to the matrices whose mean group length is at least 24 and
+ * windows to those with at least 40 (and windows), the others use direct loads. */
 int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
                     int32_t stage_lds, wk_batch **out);
 

@@ -50,8 +50,8 @@ def make_engines(ms, dtype="f64"):
     return engs
 
 
-@pytest.mark.parametrize("k,stage,wpc", [(1, 0, 0), (1, 1, 0), (1, 2, 0), (4, 2, 0), (3, 0, 0),
-                                         (1, 2, 8), (1, 0, 64), (4, 2, 16)])
+@pytest.mark.parametrize("k,stage,wpc", [(1, 0, 0), (1, 1, 0), (1, 2, 0), (1, 3, 0), (4, 3, 0),
+                                         (3, 0, 0), (1, 3, 8), (1, 0, 64), (4, 2, 16)])
 def test_batch_of_64_matrices_bitexact(k, stage, wpc):
     ms = mix()
     say(f"[gpu] batch of {len(ms)} matrices, {sum(m.nnz for m in ms)} nnz, k={k}, stage={stage}, "

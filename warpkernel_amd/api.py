@@ -493,7 +493,7 @@ class batch:
     waves_per_cu = 0: one wave per group (default); > 0: that many persistent
     waves per compute unit over equal-cost ranges."""
 
-    def __init__(self, engines, waves_per_cu: int = 0, stage_lds: int = 2):
+    def __init__(self, engines, waves_per_cu: int = 0, stage_lds: int = 3):
         self.engines = list(engines)  # keep them alive
         arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
         h = C.c_void_p()

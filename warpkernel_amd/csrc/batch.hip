@@ -35,7 +35,11 @@ struct BatchDesc {
   double *y;
   int64_t ldx, ldy;
   int32_t nznrows, nrows, ngroups, split;
-  int32_t staged;  // stream this matrix's groups through LDS (long rows)
+  int32_t staged;    // stream this matrix's groups through LDS (long rows)
+  int32_t windowed;  // and read x from the group's LDS window (stage 3)
+  const uint16_t *win_idx;
+  const uint32_t *win_off;
+  const int32_t *win_cols;
 };
 
 struct BatchItem {
@@ -58,6 +62,7 @@ struct wk_batch {
   wk::BatchItem *d_items = nullptr;
   int32_t *d_wstart = nullptr;
   int64_t nitems = 0;
+  int32_t win_bytes = 0;  // LDS window per wave (largest windowed matrix), 16-byte multiple
 };
 
 struct wk_graph {
@@ -71,10 +76,13 @@ namespace {
 // One group of one matrix for right-hand sides [j0, j0+KT): the summation
 // order of warpKernel / warpKernel2 (every slot by fma onto -0, i.e. first
 // product then fma; the pairwise lane tree for split rows).  With STAGE > 0
-// and KT = 1 the group streams through the wave's LDS slice (16-byte loads).
+// and KT = 1 the group streams through the wave's LDS slice (16-byte loads);
+// with STAGE 3 a windowed matrix reads x from the wave's LDS window (the
+// win_bytes in front of the slice).
 template <class P, int KT, int U, int STAGE>
 __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32_t j0, int32_t lane,
-                                            char *slice) {
+                                            char *region, int32_t win_bytes) {
+  char *slice = region + win_bytes;
   using V = typename P::V;
   using X = typename P::X;
   using Acc = typename P::Acc;
@@ -87,8 +95,21 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   const X *xb = static_cast<const X *>(d.x) + j0;
   Acc acc[KT];
   bool done = false;
+  if constexpr (STAGE == 3 && KT == 1) {
+    if (d.windowed) {
+      X *xs = reinterpret_cast<X *>(region);
+      const uint16_t *ibase = d.win_idx + base;
+      WinBatch<P, U> bt;
+      const bool early = len >= (uint32_t)U;
+      if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
+      const uint32_t w0 = d.win_off[g], w1 = d.win_off[g + 1];
+      fill_window<X, 1>(xs, d.win_cols + w0, w1 - w0, xb, d.ldx, lane);
+      window_group<P, U, 1>(acc, vbase, ibase, xs, len, bt, early, slice, lane);
+      done = true;
+    }
+  }
   if constexpr (STAGE > 0 && KT == 1) {
-    if (d.staged) {
+    if (!done && d.staged) {
       acc[0] = staged_group<P, U, STAGE == 2>(vbase, cbase, xb, d.ldx, len, slice, lane);
       done = true;
     }
@@ -125,10 +146,13 @@ template <class P, int KT, int U, int STAGE>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict__ desc,
                                                     const BatchItem *__restrict__ items,
                                                     const int32_t *__restrict__ wstart,
-                                                    int32_t nwaves, int32_t ktiles) {
-  // wstart == nullptr: wave w runs item w alone
+                                                    int32_t nwaves, int32_t ktiles,
+                                                    int32_t win_bytes) {
+  // wstart == nullptr: wave w runs item w alone.  Per wave, LDS holds the
+  // window (win_bytes, stage 3) and then the staging slice.
   constexpr int kSlice = (STAGE > 0 && KT == 1) ? staged_bytes<P, U>() : 16;
-  __shared__ __attribute__((aligned(16))) char lds[4 * kSlice];
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  const int32_t region = win_bytes + kSlice;
   const int32_t lane = threadIdx.x & 63;
   const int32_t wib = threadIdx.x >> 6;
   const int32_t w = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * (blockDim.x >> 6) + wib));
@@ -147,7 +171,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict_
       continue;
     }
     for (int32_t kt = 0; kt < ktiles; kt++)
-      batch_group<P, KT, U, STAGE>(d, item.g, kt * KT, lane, lds + wib * kSlice);
+      batch_group<P, KT, U, STAGE>(d, item.g, kt * KT, lane, lds + wib * region, win_bytes);
   }
 }
 
@@ -159,14 +183,19 @@ void launch_batch(const wk_batch *B, int32_t ktiles, hipStream_t s) {
   const int64_t waves = flat ? B->nitems : B->nwaves;
   const int64_t blocks = (waves + 3) / 4;
   if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
-  hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE>), dim3((uint32_t)blocks), dim3(256), 0, s,
-                     B->d_desc, B->d_items, flat ? nullptr : B->d_wstart, (int32_t)waves, ktiles);
+  constexpr int kSlice = (STAGE > 0 && KT == 1) ? staged_bytes<P, 8>() : 16;
+  const int32_t win = STAGE == 3 ? B->win_bytes : 0;
+  const size_t lds = 4 * (size_t)(win + kSlice);
+  hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE>), dim3((uint32_t)blocks), dim3(256), lds, s,
+                     B->d_desc, B->d_items, flat ? nullptr : B->d_wstart, (int32_t)waves, ktiles,
+                     win);
 }
 
 template <class P>
 BatchFn pick_batch_kt(int kt, int stage) {
   switch (kt) {
     case 1:
+      if (stage == 3) return &launch_batch<P, 1, 3>;
       if (stage == 2) return &launch_batch<P, 1, 2>;
       if (stage == 1) return &launch_batch<P, 1, 1>;
       return &launch_batch<P, 1, 0>;
@@ -198,7 +227,7 @@ int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_
     if (!out || !engines || count < 1) fail(WK_ERR_ARG, "bad argument");
     *out = nullptr;
     if (waves_per_cu < 0 || waves_per_cu > 1024) fail(WK_ERR_ARG, "waves_per_cu must be in [0, 1024]");
-    if (stage_lds < 0 || stage_lds > 2) fail(WK_ERR_ARG, "stage_lds must be 0, 1 or 2");
+    if (stage_lds < 0 || stage_lds > 3) fail(WK_ERR_ARG, "stage_lds must be 0, 1, 2 or 3");
     auto *B = new wk_batch();
     B->stage = stage_lds;
     try {
@@ -236,6 +265,7 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
     const int32_t kt = k % 8 == 0 ? 8 : k % 4 == 0 ? 4 : k % 2 == 0 ? 2 : 1;
     const int32_t ktiles = k / kt;
     std::vector<BatchItem> items;
+    int32_t win_bytes = 0;
     std::vector<int64_t> cost;
     for (size_t i = 0; i < B->engines.size(); i++) {
       const wk_engine *E = B->engines[i];
@@ -258,7 +288,17 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       d.ngroups = D.ngroups;
       d.split = D.variant == 2;
       // staging pays off for long rows only (it adds LDS traffic per batch)
-      d.staged = D.ngroups && (double)D.allocate_nz / ((double)D.ngroups * 64.0) >= 24.0;
+      const double mean_len = D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * 64.0) : 0.0;
+      d.staged = D.ngroups && mean_len >= 24.0;
+      // x windows pay off on long groups (see the single-matrix default)
+      d.windowed = B->stage == 3 && kt == 1 && D.win_max > 0 && mean_len >= 40.0;
+      d.win_idx = D.win_idx;
+      d.win_off = D.win_off;
+      d.win_cols = D.win_cols;
+      if (d.windowed) {
+        const int32_t xsz = E->dtype == WK_F64 ? 8 : 4;
+        win_bytes = std::max<int32_t>(win_bytes, (D.win_max * xsz + 15) / 16 * 16);
+      }
       if ((D.nrows && !y_devs[i]) || (E->ncols && !x_devs[i])) fail(WK_ERR_ARG, "NULL vector");
       for (int32_t g = 0; g < D.ngroups; g++) {
         items.push_back({(int32_t)i, g});
@@ -310,6 +350,15 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
     hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(), hipMemcpyHostToDevice), "bind");
     if (!wstart.empty())
       hip_check(hipMemcpy(B->d_wstart, wstart.data(), sizeof(int32_t) * wstart.size(), hipMemcpyHostToDevice), "bind");
+    // the window and the staging slice of 4 waves must fit one block's LDS
+    if (win_bytes && 4 * (size_t)(win_bytes + staged_bytes<PF64, 8>()) > 65536) {
+      for (BatchDesc &d : B->desc) d.windowed = 0;
+      win_bytes = 0;
+      hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(),
+                          hipMemcpyHostToDevice),
+                "bind");
+    }
+    B->win_bytes = win_bytes;
     B->nitems = (int64_t)items.size();
     B->k = k;
     B->kt = kt;
