@@ -307,7 +307,9 @@ def extra_device_scan(torch, dev, steps, warmup, tune=True):
             dev_s.append(time.perf_counter() - t0)
         same = all(np.array_equal(eh.download(w), ed.download(w)) for w in
                    (capi.WK_ROW_MAP, capi.WK_REORDER_ROWS, capi.WK_WARP_OFFSETS,
-                    capi.WK_ENGINE_VALUES, capi.WK_ENGINE_COLS, capi.WK_ENGINE_COLS_KR))
+                    capi.WK_ENGINE_VALUES, capi.WK_ENGINE_COLS, capi.WK_ENGINE_COLS_KR,
+                    capi.WK_ENGINE_WIN_IDX, capi.WK_ENGINE_WIN_OFF, capi.WK_ENGINE_WIN_COLS,
+                    capi.WK_ENGINE_WIN_COLS_KR))
         key = "wpk1" if variant == 1 else f"wpk2_t{th}"
         out[key] = {"host_scan_process_upload_s": round(min(host_s), 4),
                     "device_scan_process_s": round(min(dev_s), 4),

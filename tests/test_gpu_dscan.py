@@ -55,7 +55,30 @@ def compare(rp, cols, vals, variant, threshold=0, dtype="f64", kr=True):
     assert np.array_equal(e.download(capi.WK_ENGINE_COLS), pc)
     if kr:
         assert np.array_equal(e.download(capi.WK_ENGINE_COLS_KR), O.reorder_columns_coalesced(L, cols))
+    if nrows:
+        check_windows_match_host(e, rp, cols, vals, variant, threshold, dtype, kr)
     return L, e, pv, pc
+
+
+def check_windows_match_host(e, rp, cols, vals, variant, threshold, dtype, kr):
+    """The device-built x windows equal the host-built ones of the same layout
+    (the device builder leaves out layouts with a group of over 8192 slots)."""
+    from warpkernel_amd import structure, structure2
+    n = rp.size - 1
+    s = (structure(64).scan(int(rp[-1]), n, vals, rp, cols) if variant == 1 else
+         structure2(64, 32).scan(int(rp[-1]), n, vals, rp, cols, threshold))
+    h = engine(s, vals, cols, dtype=dtype, kr=kr)
+    arrays = [capi.WK_ENGINE_WIN_IDX, capi.WK_ENGINE_WIN_OFF, capi.WK_ENGINE_WIN_COLS]
+    if kr:
+        arrays.append(capi.WK_ENGINE_WIN_COLS_KR)
+    host_has = h.download(capi.WK_ENGINE_WIN_OFF).size > 0
+    dev_has = e.download(capi.WK_ENGINE_WIN_OFF).size > 0
+    if host_has and int(np.max(s.max_nz_rows)) * 64 <= 8192:
+        assert dev_has
+    if host_has and dev_has:
+        for w in arrays:
+            assert np.array_equal(e.download(w), h.download(w)), w
+    h.close()
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])

@@ -193,6 +193,148 @@ __global__ __launch_bounds__(256) void range_check_kernel(const int32_t *__restr
   if (threadIdx.x == 0) flags[blockIdx.x] = red[0];
 }
 
+// ---------------------------------------------------------------- x windows
+// The windows of build_windows (scan.cpp), built on the device: one block per
+// group sorts the group's slot keys (the entry's column, -1 for a pad) in LDS,
+// counts the distinct ones (pass 1), then writes them and each slot's
+// position among them (pass 2).  Groups of more than kWinSlots slots are not
+// handled here: the engine then has no windows.
+constexpr int32_t kWinSlots = 8192;
+constexpr int kWinThreads = 256;
+
+struct WinBuild {
+  const int32_t *rowptr, *row_map, *row_map_inv, *cols;
+  const int64_t *wo;
+  const uint32_t *aw, *tpr, *row_off;
+  int32_t nznrows;
+};
+
+// Sorted keys of group g in keys[0, N) (INT_MAX past the S slots); returns S,
+// or -1 when the group has too many slots.
+__device__ int32_t sort_group_keys(const WinBuild &b, int32_t g, int32_t *keys, int32_t *lane_n,
+                                   int32_t &N) {
+  const uint32_t A = b.aw[g];
+  const int64_t S64 = (int64_t)A * 64;
+  if (S64 > kWinSlots) return -1;
+  const int32_t S = (int32_t)S64;
+  const int32_t tid = threadIdx.x;
+  if (tid < 64) {
+    // entries of lane tid (the fill kernel's chunk rule)
+    const uint32_t tpr = b.tpr ? b.tpr[g] : 1u;
+    const uint32_t p = (b.tpr ? b.row_off[g] : (uint32_t)g * 64u) + (uint32_t)tid / tpr;
+    const uint32_t chunk = (uint32_t)tid & (tpr - 1);
+    int32_t n = 0;
+    if (p < (uint32_t)b.nznrows) {
+      const int32_t r = b.row_map[p];
+      const int32_t rb = b.rowptr[r] + (int32_t)(chunk * A);
+      n = max(0, min(b.rowptr[r + 1] - rb, (int32_t)A));
+    }
+    lane_n[tid] = n;
+  }
+  N = 64;
+  while (N < S) N <<= 1;
+  __syncthreads();
+  const int64_t base = b.wo[g];
+  for (int32_t t = tid; t < N; t += kWinThreads)
+    keys[t] = t < S ? ((t >> 6) < lane_n[t & 63] ? b.cols[base + t] : -1) : INT32_MAX;
+  __syncthreads();
+  for (int32_t k = 2; k <= N; k <<= 1)
+    for (int32_t j = k >> 1; j > 0; j >>= 1) {
+      for (int32_t t = tid; t < N; t += kWinThreads) {
+        const int32_t o = t ^ j;
+        if (o > t) {
+          const int32_t a = keys[t], c = keys[o];
+          if ((a > c) == ((t & k) == 0)) {
+            keys[t] = c;
+            keys[o] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  return S;
+}
+
+__global__ __launch_bounds__(kWinThreads) void window_count_kernel(WinBuild b, int32_t *count) {
+  __shared__ int32_t keys[kWinSlots];
+  __shared__ int32_t lane_n[64];
+  __shared__ int32_t part[kWinThreads];
+  const int32_t g = blockIdx.x;
+  int32_t N = 0;
+  const int32_t S = sort_group_keys(b, g, keys, lane_n, N);
+  if (S < 0) {
+    if (threadIdx.x == 0) count[g] = -1;
+    return;
+  }
+  int32_t c = 0;
+  for (int32_t t = threadIdx.x; t < N; t += kWinThreads)
+    c += keys[t] != INT32_MAX && (t == 0 || keys[t] != keys[t - 1]);
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = kWinThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) count[g] = part[0] > kMaxWindow ? -1 : part[0];
+}
+
+__global__ __launch_bounds__(kWinThreads) void window_write_kernel(WinBuild b, const uint32_t *off,
+                                                                   int32_t *win_cols,
+                                                                   int32_t *win_cols_kr,
+                                                                   uint16_t *win_idx) {
+  __shared__ int32_t keys[kWinSlots];
+  __shared__ int32_t uniq[kMaxWindow];
+  __shared__ int32_t lane_n[64];
+  __shared__ int32_t part[kWinThreads];
+  const int32_t g = blockIdx.x;
+  const int32_t tid = threadIdx.x;
+  int32_t N = 0;
+  const int32_t S = sort_group_keys(b, g, keys, lane_n, N);
+  if (S < 0) return;  // not reached: the host builds windows only when every group fits
+  // ranks of the distinct keys: each thread counts a contiguous chunk of the
+  // sorted keys, then an inclusive scan of the chunk counts
+  const int32_t per = N / kWinThreads > 0 ? N / kWinThreads : 1;
+  const int32_t t0 = min(N, tid * per), t1 = min(N, t0 + per);
+  int32_t c = 0;
+  for (int32_t t = t0; t < t1; t++) c += keys[t] != INT32_MAX && (t == 0 || keys[t] != keys[t - 1]);
+  part[tid] = c;
+  __syncthreads();
+  for (int o = 1; o < kWinThreads; o <<= 1) {
+    const int32_t v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int32_t rank = part[tid] - c;
+  const uint32_t w0 = off[g];
+  for (int32_t t = t0; t < t1; t++) {
+    const int32_t k = keys[t];
+    if (k != INT32_MAX && (t == 0 || k != keys[t - 1])) {
+      uniq[rank] = k;
+      win_cols[w0 + rank] = k < 0 ? 0 : k;
+      if (win_cols_kr) win_cols_kr[w0 + rank] = k < 0 ? 0 : b.row_map_inv[k];
+      rank++;
+    }
+  }
+  __syncthreads();
+  // each slot's position among the distinct keys (ascending; a pad's key -1,
+  // when present, is the first)
+  const int32_t n = part[kWinThreads - 1];
+  const int64_t base = b.wo[g];
+  for (int32_t t = tid; t < S; t += kWinThreads) {
+    const int32_t key = (t >> 6) < lane_n[t & 63] ? b.cols[base + t] : -1;
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (uniq[mid] < key)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    win_idx[base + t] = (uint16_t)lo;
+  }
+}
+
 template <class T>
 std::vector<T> download(const T *d, int64_t n, hipStream_t s) {
   std::vector<T> h((size_t)n);
@@ -328,6 +470,35 @@ void device_scan(wk_engine *E, int32_t variant, int32_t threshold, int32_t max_t
     launch_fill<float>(D, rowptr, colind, vals, s);
   }
   hip_check(hipStreamSynchronize(s), "device scan");
+
+  // 5. x windows, when every group fits
+  if (G) {
+    WinBuild wb{rowptr, D.row_map, D.row_map_inv, D.cols, D.warp_offsets, D.max_nz_rows,
+                D.tpr, D.row_off, nznrows};
+    int32_t *dcount = dev_alloc<int32_t>(E, G);
+    hipLaunchKernelGGL(window_count_kernel, dim3((uint32_t)G), dim3(kWinThreads), 0, s, wb, dcount);
+    hip_check(hipGetLastError(), "window count");
+    const std::vector<int32_t> cnt = download(dcount, G, s);
+    bool fits = true;
+    std::vector<uint32_t> off((size_t)G + 1, 0);
+    int32_t most = 0;
+    for (int32_t g = 0; g < G && fits; g++) {
+      if (cnt[g] < 0) fits = false;
+      off[g + 1] = off[g] + (uint32_t)std::max(cnt[g], 0);
+      most = std::max(most, cnt[g]);
+    }
+    if (fits) {
+      D.win_off = dev_upload(E, off.data(), (int64_t)off.size(), s);
+      D.win_cols = dev_alloc<int32_t>(E, off[G]);
+      D.win_cols_kr = kr ? dev_alloc<int32_t>(E, off[G]) : nullptr;
+      D.win_idx = dev_alloc<uint16_t>(E, D.allocate_nz);
+      hipLaunchKernelGGL(window_write_kernel, dim3((uint32_t)G), dim3(kWinThreads), 0, s, wb,
+                         D.win_off, D.win_cols, D.win_cols_kr, D.win_idx);
+      hip_check(hipGetLastError(), "window write");
+      hip_check(hipStreamSynchronize(s), "window write");
+      D.win_max = most;
+    }
+  }
 
   wk_layout_info &I = E->info;
   I.nznrows = nznrows;
