@@ -274,8 +274,11 @@ int wk_residual(wk_engine *engine, const wk_launch *cfg, const double *b_dev, in
                 const double *x_dev, double *r_dev, int32_t k, int64_t ldx, int64_t ldr,
                 void *stream);
 
-/* The backward of y = A(vals) x in one call: dx = A^T dy (as wk_run_adjoint,
- * with cfg) then dvals (as wk_dvals) on `stream`, FP64 engines.  No reference
+/* The backward of y = A(vals) x in one call: dx = A^T dy (as wk_run_adjoint)
+ * and dvals (as wk_dvals) on `stream`, FP64 engines, bit-identical to those two
+ * calls.  For one contiguous right-hand side and short groups (mean length
+ * below 40) both run in one launch whose blocks interleave the two kinds of
+ * work; otherwise two launches, the adjoint with cfg.  No reference
  * counterpart (the reference has no adjoint). */
 int wk_backward(wk_engine *engine, const wk_launch *cfg, const void *dy_dev, int64_t lddy,
                 const void *x_dev, int64_t ldx, double *dx_dev, int64_t lddx, double *dvals_dev,

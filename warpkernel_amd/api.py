@@ -392,7 +392,8 @@ class engine:
         check(lib().wk_dvals(self._h, _dev_ptr(dy), lddy, _dev_ptr(x), ldx, k, _dev_ptr(out), _stream(stream)))
 
     def backward(self, dy, x, dx, dvals, cfg=None, stream=None):
-        """dx = A^T dy and dvals in one call (FP64 engines)."""
+        """dx = A^T dy and dvals in one call (FP64 engines; one launch for a single
+        right-hand side on short groups)."""
         k, lddy = self._k_ld(dy)
         _, ldx = self._k_ld(x)
         _, lddx = self._k_ld(dx)

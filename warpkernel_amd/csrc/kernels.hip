@@ -390,6 +390,63 @@ __global__ __launch_bounds__(kDvThreads) void dvals_kernel(DvArgs d) {
   dvals_tile(d, (int32_t)blockIdx.x, srow);
 }
 
+// dx = A^T dy and dvals in one launch, for layouts of short groups: the
+// adjoint blocks (the direct body, one right-hand side, no LDS) and the dvals
+// tiles (8 KB of LDS) are spread evenly over the grid, so one kind's
+// latency-bound phases overlap the other's streaming and the launch and tail
+// costs are paid once.  Block b is dvals tile floor(b*ndv/T) when that count
+// steps at b, else adjoint block b - floor(b*ndv/T), T = nspmv + ndv.
+template <bool SPLIT>
+__device__ __forceinline__ void direct_block(const SpmvArgs &a, int32_t bid, int32_t nwork) {
+  using P = PF64;
+  using Acc = P::Acc;
+  constexpr int W = 64, U = 8;
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t wpb = blockDim.x >> 6;
+  if (bid >= nwork) {
+    const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
+    if (pos < a.nrows) a.y[(int64_t)a.row_map[pos] * a.ldy] = 0.0;
+    return;
+  }
+  const int32_t wave = __builtin_amdgcn_readfirstlane(bid * wpb + (int32_t)(threadIdx.x >> 6));
+  if (wave >= a.ngroups) return;
+  const GroupMeta m = load_meta<W, SPLIT>(a, wave);
+  const double *vp[1] = {static_cast<const double *>(a.vals) + m.base + lane};
+  const int32_t *cp[1] = {a.cols + m.base + lane};
+  const double *const(&vpc)[1] = vp;
+  const int32_t *const(&cpc)[1] = cp;
+  const double *xb = static_cast<const double *>(a.x);
+  Acc acc[1][1];
+  P::init(acc[0][0]);
+  const uint32_t i = stream_steps<P, W, 1, U, true, 1>(acc, vpc, cpc, xb, a.ldx, 0u, m.len);
+  stream_tail<P, W, 1, U, true>(acc[0], vp[0], cp[0], xb, a.ldx, i, m.len);
+  Acc r = acc[0][0];
+  if constexpr (SPLIT) {
+    for (uint32_t o = 1; o < m.tpr; o <<= 1) {
+      Acc other = P::shfl_down(r, (int)o, W);
+      P::add(r, other);
+    }
+  }
+  const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
+  if (((uint32_t)lane & (m.tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
+  a.y[(int64_t)a.row_map[rowid] * a.ldy] = P::result(r);
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(kDvThreads) void backward_kernel(SpmvArgs a, int32_t nwork,
+                                                              int32_t nspmv, DvArgs d,
+                                                              int32_t ndv) {
+  __shared__ int32_t srow[kDvTile];
+  const int64_t T = (int64_t)nspmv + ndv;
+  const int64_t b = blockIdx.x;
+  const int64_t before = b * ndv / T;
+  const int64_t upto = (b + 1) * ndv / T;
+  if (upto > before)
+    dvals_tile(d, (int32_t)before, srow);
+  else
+    direct_block<SPLIT>(a, (int32_t)(b - before), nwork);
+}
+
 // ------------------------------------------------------------- dispatch
 using LaunchFn = void (*)(const SpmvArgs &, dim3, dim3, size_t, int32_t, int32_t, int32_t,
                           hipStream_t);
@@ -741,6 +798,42 @@ void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *t
   const DvArgs d{rowptr, colind, tile_row, nnz, dy, lddy, x, ldx, k, dvals};
   hipLaunchKernelGGL(dvals_kernel, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
   hip_check(hipGetLastError(), "dvals launch");
+}
+
+void launch_backward(const DevLayout &T, const double *dy, double *dx, const int32_t *rowptr,
+                     const int32_t *colind, const int32_t *tile_row, int64_t nnz, const double *x,
+                     double *dvals, hipStream_t stream) {
+  if (T.W != 64) fail(WK_ERR_ARG, "the combined backward needs group width 64");
+  constexpr int wpb = kDvThreads / 64;
+  SpmvArgs a{};
+  a.vals = T.vals;
+  a.cols = T.cols;
+  a.row_map = T.row_map;
+  a.warp_offsets = T.warp_offsets;
+  a.max_nz_rows = T.max_nz_rows;
+  a.tpr = T.tpr;
+  a.row_off = T.row_off;
+  a.x = dy;
+  a.y = dx;
+  a.ldx = a.ldy = a.ldb = 1;
+  a.ngroups = T.ngroups;
+  a.nznrows = T.nznrows;
+  a.nrows = T.nrows;
+  a.k = 1;
+  const int64_t nwork = ((int64_t)T.ngroups + wpb - 1) / wpb;
+  const int64_t nfill = ((int64_t)T.nrows - T.nznrows + kDvThreads - 1) / kDvThreads;
+  const int64_t ndv = nnz > 0 ? (nnz + kDvTile - 1) / kDvTile : 0;
+  const int64_t total = nwork + nfill + ndv;
+  if (total > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+  if (total == 0) return;
+  const DvArgs d{rowptr, colind, tile_row, nnz, dy, 1, x, 1, 1, dvals};
+  if (T.variant == 2)
+    hipLaunchKernelGGL(backward_kernel<true>, dim3((uint32_t)total), dim3(kDvThreads), 0, stream, a,
+                       (int32_t)nwork, (int32_t)(nwork + nfill), d, (int32_t)ndv);
+  else
+    hipLaunchKernelGGL(backward_kernel<false>, dim3((uint32_t)total), dim3(kDvThreads), 0, stream,
+                       a, (int32_t)nwork, (int32_t)(nwork + nfill), d, (int32_t)ndv);
+  hip_check(hipGetLastError(), "backward launch");
 }
 
 int64_t dvals_tiles(int64_t nnz) { return (nnz + kDvTile - 1) / kDvTile; }

@@ -168,6 +168,11 @@ void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *t
                   int64_t ldx, int32_t k, double *dvals, hipStream_t stream);
 // dvals works on tiles of consecutive entries; tile_row[t] = row of entry
 // t*tile_size, tile_row[tiles] = nrows - 1.
+// dx = A^T dy and dvals in one launch (FP64, one contiguous right-hand side,
+// group width 64), for layouts whose groups are short.
+void launch_backward(const DevLayout &T, const double *dy, double *dx, const int32_t *rowptr,
+                     const int32_t *colind, const int32_t *tile_row, int64_t nnz, const double *x,
+                     double *dvals, hipStream_t stream);
 int64_t dvals_tiles(int64_t nnz);
 int64_t dvals_tile_size();
 
