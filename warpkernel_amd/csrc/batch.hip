@@ -97,7 +97,7 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   bool done = false;
   if constexpr (STAGE == 3 && KT == 1) {
     if (d.windowed) {
-      X *xs = reinterpret_cast<X *>(region);
+      char *xs = region;
       const uint16_t *ibase = d.win_idx + base;
       WinBatch<P, U> bt;
       const bool early = len >= (uint32_t)U;

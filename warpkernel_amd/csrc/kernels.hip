@@ -242,9 +242,9 @@ struct WinArgs {
   int32_t win_bytes;     // window part (16-byte multiple)
 };
 
-template <class P, bool SPLIT, int OUT, int U, int KT>
+template <class P, bool SPLIT, int OUT, int U, int KT, int D>
 __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
-                                                               int32_t remap) {
+                                                               int32_t ktiles, int32_t remap) {
   using V = typename P::V;
   using X = typename P::X;
   using Acc = typename P::Acc;
@@ -257,26 +257,40 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
     const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
     if (pos >= a.nrows) return;
     const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
-#pragma unroll
-    for (int t = 0; t < KT; t++)
+    for (int32_t t = 0; t < ktiles * KT; t++)
       a.y[dst * a.ldy + t] = OUT == OUT_RESID ? a.b[dst * a.ldb + t] : 0.0;
     return;
   }
+  // work block -> (group block, k-tile), k-tile fastest: the waves of one
+  // group's k-tiles run together and share its values and positions in L2
+  const int32_t wid = work_id(bid, nwork, remap);
+  const int32_t gb = wid / ktiles, kt = wid - gb * ktiles;
+  const int32_t j0 = kt * KT;
   const int32_t wib = threadIdx.x >> 6;
-  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(bid, nwork, remap) * wpb + wib);
+  const int32_t wave = __builtin_amdgcn_readfirstlane(gb * wpb + wib);
   if (wave >= a.ngroups) return;
   const GroupMeta m = load_meta<W, SPLIT>(a, wave);
   const V *vbase = static_cast<const V *>(a.vals) + m.base;
   const uint16_t *ibase = w.idx + m.base;
-  char *mine = lds + wib * w.wave_bytes;
-  X *xs = reinterpret_cast<X *>(mine);
-  WinBatch<P, U> bt;
-  const bool early = m.len >= (uint32_t)U;
-  if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
+  char *xs = lds + wib * w.wave_bytes;
   const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
-  fill_window<X, KT>(xs, w.cols + w0, w1 - w0, static_cast<const X *>(a.x), a.ldx, lane);
+  const X *xk = static_cast<const X *>(a.x) + j0;
   Acc acc[KT];
-  window_group<P, U, KT>(acc, vbase, ibase, xs, m.len, bt, early, mine + w.win_bytes, lane);
+  // window columns first, then the first value batches, then the x gathers
+  WinFill<X, KT> f;
+  f.cols(w.cols + w0, w1 - w0, lane);
+  if constexpr (D == 1) {
+    WinBatch<P, U> bt;
+    const bool early = m.len >= (uint32_t)U;
+    if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, lane);
+    window_group<P, U, KT>(acc, vbase, ibase, xs, m.len, bt, early, xs + w.win_bytes, lane);
+  } else {
+    WinPipe<P, U, D> pp;
+    pp.prologue(vbase, ibase, m.len, lane);  // in flight while the window fills
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, lane);
+    window_group_deep<P, U, KT, D>(acc, vbase, ibase, xs, m.len, pp, xs + w.win_bytes, lane);
+  }
   if constexpr (SPLIT) {
     for (uint32_t o = 1; o < m.tpr; o <<= 1) {
 #pragma unroll
@@ -293,9 +307,9 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   for (int t = 0; t < KT; t++) {
     const double res = P::result(acc[t]);
     if constexpr (OUT == OUT_RESID)
-      a.y[dst * a.ldy + t] = a.b[dst * a.ldb + t] - res;
+      a.y[dst * a.ldy + j0 + t] = a.b[dst * a.ldb + j0 + t] - res;
     else
-      a.y[dst * a.ldy + t] = res;
+      a.y[dst * a.ldy + j0 + t] = res;
   }
 }
 
@@ -575,27 +589,28 @@ LdsLimits lds_limits() {
 }
 
 using WinFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
-                       hipStream_t);
+                       int32_t, hipStream_t);
 
-template <class P, bool SPLIT, int OUT, int U, int KT>
+template <class P, bool SPLIT, int OUT, int U, int KT, int D>
 void launch_win(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
-                int32_t nwork, int32_t remap, hipStream_t s) {
-  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U, KT>), grid, block, lds, s, a, w, nwork,
-                     remap);
+                int32_t nwork, int32_t ktiles, int32_t remap, hipStream_t s) {
+  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U, KT, D>), grid, block, lds, s, a, w, nwork,
+                     ktiles, remap);
 }
 
-// k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8
+// k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8 and
+// two batches in flight (measured best of one, two and four)
 template <class P, bool SPLIT, int OUT>
 WinFn pick_win_u(int u, int kt) {
   if (kt == 1) {
-    if (u == 8) return &launch_win<P, SPLIT, OUT, 8, 1>;
-    if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1>;
+    if (u == 8) return &launch_win<P, SPLIT, OUT, 8, 1, 1>;
+    if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1, 1>;
     return nullptr;
   }
   if (u != 8) return nullptr;
-  if (kt == 2) return &launch_win<P, SPLIT, OUT, 8, 2>;
-  if (kt == 4) return &launch_win<P, SPLIT, OUT, 8, 4>;
-  if (kt == 8) return &launch_win<P, SPLIT, OUT, 8, 8>;
+  if (kt == 2) return &launch_win<P, SPLIT, OUT, 8, 2, 2>;
+  if (kt == 4) return &launch_win<P, SPLIT, OUT, 8, 4, 2>;
+  if (kt == 8) return &launch_win<P, SPLIT, OUT, 8, 8, 2>;
   return nullptr;
 }
 
@@ -703,18 +718,16 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         w.win_bytes = (int32_t)win_bytes;
         const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
         const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
-        if (nb + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
-        for (int32_t t = 0; t < ntiles; t++) {
-          const int32_t jt = j0 + t * kt;
-          a.x = static_cast<const char *>(x) + (size_t)jt * xsz;
-          a.y = y + jt;
-          a.b = b ? b + jt : nullptr;
-          a.k = kt;
-          if (nb + nf > 0)
-            wf(a, w, dim3((uint32_t)(nb + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nb,
-               cfg.xcd_remap ? 1 : 0, stream);
-          hip_check(hipGetLastError(), "spmv launch");
-        }
+        const int64_t nwork = nb * ntiles;
+        if (nwork + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+        a.x = static_cast<const char *>(x) + (size_t)j0 * xsz;
+        a.y = y + j0;
+        a.b = b ? b + j0 : nullptr;
+        a.k = ntiles * kt;
+        if (nwork + nf > 0)
+          wf(a, w, dim3((uint32_t)(nwork + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nwork,
+             ntiles, cfg.xcd_remap ? 1 : 0, stream);
+        hip_check(hipGetLastError(), "spmv launch");
         j0 += ntiles * kt;
         continue;
       }

@@ -319,40 +319,115 @@ struct WinBatch {
   }
 };
 
-// Fills the window: xs[j*KT + t] = x[wcols[j] * ldx + t], j < n, t < KT,
-// 8 entries per lane in flight at a time.
+// One window entry: the KT values of x at one column, kBytes = KT*sizeof(X)
+// bytes at j*kBytes.  LDS has 32 four-byte banks; with 32- or 64-byte entries
+// a 16-byte read at the same chunk of every lane's entry lands on only 16 or
+// 8 of them (entry j starts at bank 8j or 16j mod 32), so such entries store
+// their 16-byte chunks in an order that depends on j: chunk q sits at
+// position q ^ swz(j), and the reads of 64 lanes at scattered entries spread
+// over all 32 banks.  Entries of 4 to 16 bytes are stored plainly.
 template <class X, int KT>
-__device__ __forceinline__ void fill_window(X *xs, const int32_t *__restrict__ wcols, uint32_t n,
-                                            const X *__restrict__ xb, int64_t ldx, int32_t lane) {
-  constexpr int C = KT >= 4 ? 4 : 8;
-  for (uint32_t j0 = 0; j0 < n; j0 += C * 64) {
-    int32_t c[C];
+struct WinEntry {
+  static constexpr uint32_t kBytes = KT * (uint32_t)sizeof(X);
+  static constexpr uint32_t kChunks = kBytes >= 32 ? kBytes / 16 : 1;
+  static constexpr int kPer = 16 / (int)sizeof(X);  // values per chunk
+  // entry j's first bank is (kBytes/4 * j) mod 32, which repeats every
+  // 8/kChunks entries: the next bits of j pick the chunk order
+  static constexpr int kShift = kChunks == 2 ? 2 : kChunks == 4 ? 1 : 0;
+  static_assert(kChunks == 1 || kChunks == 2 || kChunks == 4 || kChunks == 8, "entry size");
+  __device__ static __forceinline__ uint32_t swz(uint32_t j) {
+    return kChunks > 1 ? (j >> kShift) & (kChunks - 1) : 0u;
+  }
+  __device__ static __forceinline__ void read(const char *xs, uint32_t j, X (&v)[KT]) {
+    if constexpr (kChunks == 1) {
+      const X *e = reinterpret_cast<const X *>(xs + j * kBytes);
 #pragma unroll
-    for (int q = 0; q < C; q++) {
-      const uint32_t j = j0 + q * 64 + lane;
-      c[q] = j < n ? wcols[j] : 0;
-    }
-    X v[C][KT];
+      for (int t = 0; t < KT; t++) v[t] = e[t];
+    } else {
+      const uint32_t base = j * kBytes, s = swz(j);
 #pragma unroll
-    for (int q = 0; q < C; q++)
-#pragma unroll
-      for (int t = 0; t < KT; t++) v[q][t] = xb[(int64_t)c[q] * ldx + t];
-#pragma unroll
-    for (int q = 0; q < C; q++) {
-      const uint32_t j = j0 + q * 64 + lane;
-      if (j < n)
-#pragma unroll
-        for (int t = 0; t < KT; t++) xs[(int64_t)j * KT + t] = v[q][t];
+      for (uint32_t q = 0; q < kChunks; q++) {
+        const u32x4 c = *reinterpret_cast<const u32x4 *>(xs + base + 16u * (q ^ s));
+        __builtin_memcpy(&v[q * kPer], &c, 16);
+      }
     }
   }
+  __device__ static __forceinline__ void write(char *xs, uint32_t j, const X (&v)[KT]) {
+    if constexpr (kChunks == 1) {
+      X *e = reinterpret_cast<X *>(xs + j * kBytes);
+#pragma unroll
+      for (int t = 0; t < KT; t++) e[t] = v[t];
+    } else {
+      const uint32_t base = j * kBytes, s = swz(j);
+#pragma unroll
+      for (uint32_t q = 0; q < kChunks; q++) {
+        u32x4 c;
+        __builtin_memcpy(&c, &v[q * kPer], 16);
+        *reinterpret_cast<u32x4 *>(xs + base + 16u * (q ^ s)) = c;
+      }
+    }
+  }
+};
+
+// The window fill in two halves, so that the column loads go out before a
+// group's first value batches and the x gathers right after them: cols()
+// loads the first C*64 window columns (C per lane), gather() reads x at them,
+// writes those entries, and repeats for any further columns.  C keeps the
+// gathered values of one round within about 96 registers.
+template <class X, int KT>
+struct WinFill {
+  static constexpr int kRegs = KT * (int)sizeof(X) / 4;  // registers per entry
+  static constexpr int C = 96 / kRegs > 12 ? 12 : (96 / kRegs < 4 ? 4 : 96 / kRegs);
+  int32_t c[C];
+  __device__ __forceinline__ void cols(const int32_t *__restrict__ wcols, uint32_t n,
+                                       int32_t lane) {
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const uint32_t j = q * 64 + lane;
+      c[q] = j < n ? wcols[j] : 0;
+    }
+  }
+  __device__ __forceinline__ void gather(char *xs, const int32_t *__restrict__ wcols, uint32_t n,
+                                         const X *__restrict__ xb, int64_t ldx, int32_t lane) {
+    for (uint32_t j0 = 0;;) {
+      X v[C][KT];
+#pragma unroll
+      for (int q = 0; q < C; q++)
+#pragma unroll
+        for (int t = 0; t < KT; t++) v[q][t] = xb[(int64_t)c[q] * ldx + t];
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const uint32_t j = j0 + q * 64 + lane;
+        if (j < n) WinEntry<X, KT>::write(xs, j, v[q]);
+      }
+      j0 += C * 64;
+      if (j0 >= n) break;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const uint32_t j = j0 + q * 64 + lane;
+        c[q] = j < n ? wcols[j] : 0;
+      }
+    }
+  }
+};
+
+// Fills the window in one go: entry j < n holds x[wcols[j] * ldx + t], t < KT.
+template <class X, int KT>
+__device__ __forceinline__ void fill_window(char *xs, const int32_t *__restrict__ wcols, uint32_t n,
+                                            const X *__restrict__ xb, int64_t ldx, int32_t lane) {
+  WinFill<X, KT> f;
+  f.cols(wcols, n, lane);
+  f.gather(xs, wcols, n, xb, ldx, lane);
 }
 
 template <class P, int U, int KT>
 __device__ __forceinline__ void window_group(typename P::Acc (&acc)[KT], const typename P::V *vbase,
-                                             const uint16_t *ibase, const typename P::X *xs,
-                                             uint32_t len, WinBatch<P, U> &bt, bool issued,
-                                             char *slice, int32_t lane) {
+                                             const uint16_t *ibase, const char *xs, uint32_t len,
+                                             WinBatch<P, U> &bt, bool issued, char *slice,
+                                             int32_t lane) {
   using V = typename P::V;
+  using X = typename P::X;
+  using E = WinEntry<X, KT>;
   const V *lv = reinterpret_cast<const V *>(slice);
   const uint16_t *li = reinterpret_cast<const uint16_t *>(slice + U * 64 * sizeof(V));
 #pragma unroll
@@ -364,23 +439,27 @@ __device__ __forceinline__ void window_group(typename P::Acc (&acc)[KT], const t
       bt.stage(slice, lane);
       uint32_t p[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) p[u] = (uint32_t)li[u * 64 + lane] * KT;
+      for (int u = 0; u < U; u++) p[u] = li[u * 64 + lane];
       bt.issue(vbase, ibase, i + U, lane);
       V v[U];
 #pragma unroll
       for (int u = 0; u < U; u++) v[u] = lv[u * 64 + lane];
 #pragma unroll
-      for (int u = 0; u < U; u++)
+      for (int u = 0; u < U; u++) {
+        X xv[KT];
+        E::read(xs, p[u], xv);
 #pragma unroll
-        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xs[p[u] + t]);
+        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xv[t]);
+      }
     }
     bt.stage(slice, lane);
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const uint32_t p = (uint32_t)li[u * 64 + lane] * KT;
       const V v = lv[u * 64 + lane];
+      X xv[KT];
+      E::read(xs, li[u * 64 + lane], xv);
 #pragma unroll
-      for (int t = 0; t < KT; t++) P::step(acc[t], v, xs[p + t]);
+      for (int t = 0; t < KT; t++) P::step(acc[t], v, xv[t]);
     }
     i += U;
   }
@@ -396,9 +475,113 @@ __device__ __forceinline__ void window_group(typename P::Acc (&acc)[KT], const t
       }
 #pragma unroll
     for (int u = 0; u < U - 1; u++)
-      if (i + u < len)
+      if (i + u < len) {
+        X xv[KT];
+        E::read(xs, c[u], xv);
 #pragma unroll
-        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xs[(uint32_t)c[u] * KT + t]);
+        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xv[t]);
+      }
+  }
+}
+
+// The x reads and FMAs of one staged batch: slot u of the lane reads its
+// value and window position from the slice, then its KT x values.
+template <class P, int U, int KT>
+__device__ __forceinline__ void window_batch(typename P::Acc (&acc)[KT], const char *xs,
+                                             const char *slice, int32_t lane) {
+  using V = typename P::V;
+  using X = typename P::X;
+  const V *lv = reinterpret_cast<const V *>(slice);
+  const uint16_t *li = reinterpret_cast<const uint16_t *>(slice + U * 64 * sizeof(V));
+  uint32_t p[U];
+  V v[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    p[u] = li[u * 64 + lane];
+    v[u] = lv[u * 64 + lane];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    X xv[KT];
+    WinEntry<X, KT>::read(xs, p[u], xv);
+#pragma unroll
+    for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xv[t]);
+  }
+}
+
+// window_group with D batches in flight instead of one.  With KT values per
+// slot the window takes KT times the LDS, so fewer waves share a CU, and
+// each wave must keep more of the value stream in flight by itself: batch b
+// is staged and consumed while batches b+1 .. b+D-1 are loading, and its
+// registers take batch b+D right after staging.  Before the window fill the
+// caller issues batches 0 .. D-1 (prologue), so they load during the fill.
+// Same products, same order.
+template <class P, int U, int D>
+struct WinPipe {
+  WinBatch<P, U> bt[D];
+  __device__ __forceinline__ void prologue(const typename P::V *vbase, const uint16_t *ibase,
+                                           uint32_t len, int32_t lane) {
+    const uint32_t nb = len / U;
+#pragma unroll
+    for (int d = 0; d < D; d++)
+      if ((uint32_t)d < nb) bt[d].issue(vbase, ibase, (uint32_t)d * U, lane);
+  }
+};
+
+template <class P, int U, int KT, int D>
+__device__ __forceinline__ void window_group_deep(typename P::Acc (&acc)[KT],
+                                                  const typename P::V *vbase,
+                                                  const uint16_t *ibase, const char *xs,
+                                                  uint32_t len, WinPipe<P, U, D> &pp, char *slice,
+                                                  int32_t lane) {
+  using V = typename P::V;
+  using X = typename P::X;
+#pragma unroll
+  for (int t = 0; t < KT; t++) P::init(acc[t]);  // every slot by fma onto -0
+  const uint32_t nb = len / U;
+  uint32_t b = 0;
+  // steady state: the batch D ahead of each consumed one exists
+  for (; b + 2 * D <= nb; b += D) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      pp.bt[d].stage(slice, lane);
+      pp.bt[d].issue(vbase, ibase, (b + D + d) * U, lane);
+      window_batch<P, U, KT>(acc, xs, slice, lane);
+    }
+  }
+  // drain: fewer than 2D batches left
+#pragma unroll
+  for (int d = 0; d < D; d++)
+    if (b + d < nb) {
+      pp.bt[d].stage(slice, lane);
+      if (b + D + d < nb) pp.bt[d].issue(vbase, ibase, (b + D + d) * U, lane);
+      window_batch<P, U, KT>(acc, xs, slice, lane);
+    }
+#pragma unroll
+  for (int d = 0; d < D; d++)
+    if (b + D + d < nb) {
+      pp.bt[d].stage(slice, lane);
+      window_batch<P, U, KT>(acc, xs, slice, lane);
+    }
+  // the last, partial batch: direct loads, all issued before the first use
+  const uint32_t i = nb * U;
+  if (i < len) {
+    V v[U];
+    uint16_t c[U];
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) {
+        c[u] = ibase[(int64_t)(i + u) * 64 + lane];
+        v[u] = __builtin_nontemporal_load(vbase + (int64_t)(i + u) * 64 + lane);
+      }
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) {
+        X xv[KT];
+        WinEntry<X, KT>::read(xs, c[u], xv);
+#pragma unroll
+        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xv[t]);
+      }
   }
 }
 

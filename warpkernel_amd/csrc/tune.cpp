@@ -80,7 +80,7 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
 
 }  // namespace
 
-wk_launch heuristic_launch(const DevLayout &D, int32_t k) {
+wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
   wk_launch c;
   wk_launch_default(&c);
   const double mean_len =
@@ -99,15 +99,18 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t k) {
       c.unroll = 16;
       c.waves_per_block = 1;
     }
-  } else if (k >= 2 && k <= 4 && D.W == 64 && mean_len >= 40.0 && D.win_max > 0) {
-    // two to four right-hand sides in one windowed pass (window = distinct
-    // columns x k values); measured 22-43% over the direct kernel on hex
-    // elasticity, while wider windows cost more occupancy than they save
+  } else if (k >= 2 && D.W == 64 && mean_len >= 40.0 && D.win_max > 0) {
+    // several right-hand sides through windows of k-tile values per column,
+    // all k-tiles in one launch, a group's tiles on one XCD (remap) so its
+    // values and positions are read from HBM once.  Measured on hex
+    // elasticity 32^3 and 64^3 against the direct kernel: FP64 k=4 15-30%,
+    // k=8 20%, k=32 30% faster with k-tile 8; FP32 is bound by the
+    // compensated arithmetic and does best with k-tile 4.
     c.stage_lds = 3;
     c.unroll = 8;
-    c.k_tile = pow2_at_most(k);
-    c.waves_per_block = 2;
-    c.xcd_remap = 0;
+    c.k_tile = std::min(dtype == WK_F64 ? 8 : 4, pow2_at_most(k));
+    c.waves_per_block = 1;
+    c.xcd_remap = 1;
   } else if (k == 1 && D.W == 64 && mean_len >= 12.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
     // (measured best on hex diffusion and P1 tets)
@@ -131,7 +134,7 @@ wk_launch engine_launch(wk_engine *E, int32_t op, int32_t k) {
     auto it = E->launch_cfg.find(cfg_key(op, k));
     if (it != E->launch_cfg.end()) return it->second;
   }
-  return heuristic_launch(layout_for(E, op), k);
+  return heuristic_launch(layout_for(E, op), E->dtype, k);
 }
 
 }  // namespace wk

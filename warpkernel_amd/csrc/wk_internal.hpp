@@ -179,7 +179,7 @@ int64_t dvals_tile_size();
 // The configuration an engine uses for (op, k) when the caller passes none:
 // the tuned one if any, else a choice from the layout.
 wk_launch engine_launch(wk_engine *E, int32_t op, int32_t k);
-wk_launch heuristic_launch(const DevLayout &D, int32_t k);
+wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k);
 
 // WPK2's group split (warpkernel.hpp:633-691) over the longest row of each
 // W-row group of sorted rows (aw0); nzr = non-empty rows.  Outputs per
