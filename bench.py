@@ -52,6 +52,20 @@ def time_launches(torch, fn, steps, warmup):
     return wall, ev0.elapsed_time(ev1) / steps
 
 
+def stream_read_ceiling(torch, gib=4, reps=10):
+    """The device's streaming-read rate (wk_stream_read over a 4 GiB buffer,
+    16-byte nontemporal loads): the practical ceiling of the kernels' HBM
+    traffic, reported next to the nominal peak."""
+    from warpkernel_amd._capi import check, lib
+    nbytes = gib << 30
+    buf = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    _, ms = time_launches(torch, lambda: check(lib().wk_stream_read(buf.data_ptr(), nbytes, None, st)),
+                          reps, 3)
+    del buf
+    return round(nbytes / (ms * 1e-3) / 1e9, 1)
+
+
 def read_traffic(workload):
     """HBM bytes per launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
@@ -408,6 +422,13 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
         },
+    }
+    # measured context for the nominal peak: the box's streaming-read rate, and
+    # the real HBM rate of the kernel (counter traffic over the live duration)
+    ceiling = stream_read_ceiling(torch)
+    line["hbm_measured"] = {
+        "stream_read_GBs": ceiling,
+        "kernel_traffic_GBs": round(traffic / (ev_ms * 1e-3) / 1e9, 1) if traffic else None,
     }
     if not args.headline_only:
         dxr = dev(s.reorder_x(x))

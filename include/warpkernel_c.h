@@ -248,8 +248,10 @@ typedef struct wk_launch {
                               group's distinct columns with 16-bit positions
                               per slot (engines whose groups have at most 2048
                               distinct columns; otherwise as 2).  Applies to
-                              k_tile 1, group width 64, rows_per_lane 1, unroll
-                              8/16, nontemporal 1; otherwise ignored */
+                              group width 64, rows_per_lane 1, nontemporal 1,
+                              and k_tile 1 with unroll 8/16; stage 3 also to
+                              k_tiles 2, 4 and 8 (unroll 8), all k-tiles in
+                              one launch; otherwise ignored */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);
@@ -344,7 +346,8 @@ typedef struct wk_batch wk_batch;
  * in wk_launch, for batches with one right-hand side (0 direct loads, 1
  * staged, 2 staged with the next batch in flight, 3 as 2 plus x windows);
  * staging applies to the matrices whose mean group length is at least 24 and
- * windows to those with at least 40 (and windows), the others use direct loads. */
+ * windows to those with windows whose entries serve at least 4.5 slots each
+ * on average; the others use direct loads. */
 int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
                     int32_t stage_lds, wk_batch **out);
 
@@ -377,6 +380,14 @@ void wk_graph_destroy(wk_graph *graph);
 int wk_time_run(wk_engine *engine, const wk_launch *cfg, int32_t which, const void *x_dev,
                 double *y_dev, int32_t k, int64_t ldx, int64_t ldy, void *stream,
                 float *seconds);
+
+/* Measurement helper, no counterpart in the reference: reads `bytes` bytes
+ * of device memory at `buf` (16-byte aligned, size a multiple of 16) with
+ * 16-byte nontemporal loads, eight per thread in flight, on `stream`.  Its
+ * duration gives the device's streaming-read rate, the practical ceiling of
+ * the SpMV kernels' HBM traffic.  `sink` (4 bytes of device memory, or NULL)
+ * is written only if the XOR of the data hits a fixed value. */
+int wk_stream_read(const void *buf, int64_t bytes, uint32_t *sink, void *stream);
 
 #ifdef __cplusplus
 }

@@ -34,3 +34,21 @@ def test_tune_layout_returns_a_working_engine():
         L = O.scan2(m.rowptr, best.threshold, 64, 32)
         pv, pc = O.process(L, m.vals, m.colind)
         assert np.array_equal(y.cpu().numpy(), O.wpk_spmv(L, pv, pc, x.cpu().numpy()))
+
+
+@pytest.mark.gpu
+def test_stream_read_helper():
+    """The streaming-read measurement helper runs on aligned buffers, leaves the
+    sink alone unless the data's XOR hits its fixed value, and refuses sizes
+    that are not 16-byte multiples before launching."""
+    import torch
+    from warpkernel_amd._capi import WK_ERR_ARG, WK_OK, lib
+    print("[gpu] stream read helper", flush=True)
+    buf = torch.zeros(1 << 20, dtype=torch.float64, device="cuda")
+    sink = torch.full((4,), 7, dtype=torch.int32, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    assert lib().wk_stream_read(buf.data_ptr(), buf.numel() * 8, sink.data_ptr(), st) == WK_OK
+    assert lib().wk_stream_read(buf.data_ptr(), 0, None, st) == WK_OK
+    torch.cuda.synchronize()
+    assert sink.tolist() == [7, 7, 7, 7]  # XOR of zeros is 0, not the fixed value
+    assert lib().wk_stream_read(buf.data_ptr(), 24, None, st) == WK_ERR_ARG

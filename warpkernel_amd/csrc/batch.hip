@@ -290,8 +290,10 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       // staging pays off for long rows only (it adds LDS traffic per batch)
       const double mean_len = D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * 64.0) : 0.0;
       d.staged = D.ngroups && mean_len >= 24.0;
-      // x windows pay off on long groups (see the single-matrix default)
-      d.windowed = B->stage == 3 && kt == 1 && D.win_max > 0 && mean_len >= 40.0;
+      // x windows pay off where each entry serves enough slots (the rule of
+      // the single-matrix default)
+      d.windowed = B->stage == 3 && kt == 1 && D.win_max > 0 && D.win_total > 0 &&
+                   (double)D.allocate_nz >= 4.5 * (double)D.win_total;
       d.win_idx = D.win_idx;
       d.win_off = D.win_off;
       d.win_cols = D.win_cols;

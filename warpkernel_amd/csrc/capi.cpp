@@ -93,6 +93,7 @@ void upload_layout(wk_engine *E, DevLayout &D, const wk_layout &L, const double 
       D.win_idx = dev_upload(E, idx.data(), (int64_t)idx.size(), s);
       D.win_off = dev_upload(E, off.data(), (int64_t)off.size(), s);
       D.win_cols = dev_upload(E, wc.data(), (int64_t)wc.size(), s);
+      D.win_total = (int64_t)wc.size();
       hip_check(hipStreamSynchronize(s), "upload");
       if (kr) {
         for (size_t j = 0; j < wkeys.size(); j++) wc[j] = wkeys[j] < 0 ? 0 : L.row_map_inv[wkeys[j]];
@@ -377,6 +378,7 @@ int wk_engine_set_colinds(wk_engine *E, const int32_t *pcols, int32_t where, voi
     // the x windows describe the old columns: launches fall back to staging
     // without windows from here on
     E->fwd.win_max = 0;
+    E->fwd.win_total = 0;
     if (!where) hip_check(hipStreamSynchronize(s), "set_colinds");
   });
 }
@@ -498,6 +500,13 @@ int wk_run_assembly(wk_engine *E, const wk_launch *cfg, double *y_dev, void *str
     if (!y_dev && E->nrows) fail(WK_ERR_ARG, "NULL vector");
     launch_rowsum(E->fwd, E->dtype, resolve(E, WK_OP_RUN, 1, cfg), y_dev,
                   static_cast<hipStream_t>(stream));
+  });
+}
+
+int wk_stream_read(const void *buf, int64_t bytes, uint32_t *sink, void *stream) {
+  return guarded([&] {
+    if (!buf && bytes > 0) fail(WK_ERR_ARG, "NULL buffer");
+    launch_stream_read(buf, bytes, sink, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -497,6 +497,7 @@ void device_scan(wk_engine *E, int32_t variant, int32_t threshold, int32_t max_t
       hip_check(hipGetLastError(), "window write");
       hip_check(hipStreamSynchronize(s), "window write");
       D.win_max = most;
+      D.win_total = off[G];
     }
   }
 

@@ -85,21 +85,23 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
   wk_launch_default(&c);
   const double mean_len =
       D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * D.W) : 0.0;
-  if (k == 1 && D.W == 64 && mean_len >= 40.0 && D.win_max > 0) {
-    // long groups: x windows in LDS, 10 bytes per slot (measured best on hex
-    // elasticity, 81-entry rows: 11% over staging without windows); the
-    // window fill costs too much per group for short rows
+  // x windows pay where each window entry serves enough slots to amortise its
+  // fill: hex elasticity (7.6 slots per entry for WPK1, 6.0 for WPK2 at
+  // threshold 27) gains 10-30%; hex diffusion (3.4), P1 tets (2.1) and the
+  // 2D Laplacian (2.9) lose
+  const bool windows = D.W == 64 && D.win_max > 0 && D.win_total > 0 &&
+                       (double)D.allocate_nz >= 4.5 * (double)D.win_total;
+  if (k == 1 && windows) {
+    // 10 bytes per slot, x from the group's window in LDS; one wave per
+    // block, a contiguous range of groups per XCD, and 16-step batches for
+    // long groups (hex elasticity 64^3: 99.3 us against 104 with 8-step
+    // batches and no remap; WPK2 at threshold 27, 21-step groups: 8-step
+    // batches, 109.7 us against 139 with 16)
     c.stage_lds = 3;
-    c.unroll = 8;
-    c.waves_per_block = 2;
-    c.xcd_remap = 0;
-    if (D.ngroups < 2048) {
-      // few groups: each wave's steps are the critical path, so fewer, larger
-      // batches (measured 18% on 733 groups of 81-entry rows)
-      c.unroll = 16;
-      c.waves_per_block = 1;
-    }
-  } else if (k >= 2 && D.W == 64 && mean_len >= 40.0 && D.win_max > 0) {
+    c.unroll = mean_len >= 40.0 ? 16 : 8;
+    c.waves_per_block = 1;
+    c.xcd_remap = 1;
+  } else if (k >= 2 && windows) {
     // several right-hand sides through windows of k-tile values per column,
     // all k-tiles in one launch, a group's tiles on one XCD (remap) so its
     // values and positions are read from HBM once.  Measured on hex
@@ -111,20 +113,22 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     c.k_tile = std::min(dtype == WK_F64 ? 8 : 4, pow2_at_most(k));
     c.waves_per_block = 1;
     c.xcd_remap = 1;
-  } else if (k == 1 && D.W == 64 && mean_len >= 12.0) {
+  } else if (k == 1 && D.W == 64 && mean_len >= 20.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
-    // (measured best on hex diffusion and P1 tets)
+    // (measured best on hex diffusion, 27-entry rows)
     c.stage_lds = 2;
     c.unroll = 8;
     c.waves_per_block = 4;
     c.xcd_remap = 0;
   } else {
+    // short rows (P1 tets, 15 entries: 11% faster than staging)
     c.stage_lds = 0;
     c.unroll = 8;
     c.waves_per_block = 2;
     c.xcd_remap = 1;
     c.k_tile = std::min(8, pow2_at_most(std::max(1, k)));
   }
+  c.k_tile = std::min(c.k_tile, pow2_at_most(std::max(1, k)));  // as the launcher uses it
   return c;
 }
 

@@ -92,6 +92,7 @@ struct DevLayout {
   int32_t *win_cols = nullptr;     // [win_off[ngroups]]
   int32_t *win_cols_kr = nullptr;  // the same columns in Kr numbering (row_map_inv)
   int32_t win_max = 0;             // largest window; 0 when the layout has none
+  int64_t win_total = 0;           // win_off[ngroups]: entries over all windows
 };
 
 // Largest x window (distinct columns of one group) that the window kernel
@@ -155,6 +156,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
 // Zero-fill then scatter CSR values into the padded slots (GPU process).
 void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
                            hipStream_t stream);
+void launch_stream_read(const void *p, int64_t bytes, uint32_t *sink, hipStream_t stream);
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
 // Fails with WK_ERR_ARG unless every a[i], i < n, lies in [0, bound); device
 // array, synchronises the stream.
