@@ -232,7 +232,12 @@ int wk_engine_download(const wk_engine *engine, int32_t which, void *host_dst, i
 
 /* Launch options; wk_launch_default fills the built-in choice. */
 typedef struct wk_launch {
-  int32_t waves_per_block; /* 1, 2, 4 or 8 (the drivers' warps/block sweep) */
+  int32_t waves_per_block; /* 1, 2, 4 or 8 (the drivers' warps/block sweep).
+                              With stage_lds 3 and a k-tile above 1, the
+                              block's waves share one group's x window, each
+                              summing an equal share of the k-tile's
+                              right-hand sides (at most one wave per 16
+                              bytes of a window entry) */
   int32_t unroll;          /* slots in flight per lane: 4, 8 or 16 (k_tile 1) */
   int32_t k_tile;          /* right-hand sides per pass: 1, 2, 4, 8, 16 or 32 */
   int32_t xcd_remap;       /* 1: contiguous group ranges per XCD */

@@ -242,13 +242,22 @@ struct WinArgs {
   int32_t win_bytes;     // window part (16-byte multiple)
 };
 
-template <class P, bool SPLIT, int OUT, int U, int KT, int D>
+// Work block -> (group block, k-tile), k-tile fastest: the blocks of one
+// group's k-tiles run together and share its values and positions in L2.
+// WPW = 1: each wave of the block takes its own group and window.  WPW > 1
+// (k-tiles above 1): the block's WPW waves take one group, fill its window
+// together (one barrier), and wave w sums values [w*KT/WPW, (w+1)*KT/WPW) of
+// the k-tile, so a window serves WPW waves.  LDS: per wave [window | slice]
+// (WPW = 1), or [window | slice of wave 0 | ... ] (WPW > 1).
+template <class P, bool SPLIT, int OUT, int U, int KT, int D, int WPW>
 __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
                                                                int32_t ktiles, int32_t remap) {
   using V = typename P::V;
   using X = typename P::X;
   using Acc = typename P::Acc;
   constexpr int W = 64;
+  constexpr int KW = KT / WPW;  // right-hand sides per wave
+  static_assert(KW * WPW == KT, "waves split the k-tile evenly");
   extern __shared__ __attribute__((aligned(16))) char lds[];
   const int32_t lane = threadIdx.x & 63;
   const int32_t wpb = blockDim.x >> 6;
@@ -261,40 +270,44 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
       a.y[dst * a.ldy + t] = OUT == OUT_RESID ? a.b[dst * a.ldb + t] : 0.0;
     return;
   }
-  // work block -> (group block, k-tile), k-tile fastest: the waves of one
-  // group's k-tiles run together and share its values and positions in L2
   const int32_t wid = work_id(bid, nwork, remap);
   const int32_t gb = wid / ktiles, kt = wid - gb * ktiles;
-  const int32_t j0 = kt * KT;
   const int32_t wib = threadIdx.x >> 6;
-  const int32_t wave = __builtin_amdgcn_readfirstlane(gb * wpb + wib);
-  if (wave >= a.ngroups) return;
+  // the group, and the first of this wave's right-hand sides in the tile
+  const int32_t wave = __builtin_amdgcn_readfirstlane(WPW > 1 ? gb : gb * wpb + wib);
+  const uint32_t t0 = WPW > 1 ? (uint32_t)wib * KW : 0u;
+  const int32_t j0 = kt * KT;
+  if (wave >= a.ngroups) return;  // WPW > 1: the same for the whole block
   const GroupMeta m = load_meta<W, SPLIT>(a, wave);
   const V *vbase = static_cast<const V *>(a.vals) + m.base;
   const uint16_t *ibase = w.idx + m.base;
-  char *xs = lds + wib * w.wave_bytes;
+  char *xs = WPW > 1 ? lds : lds + wib * w.wave_bytes;
+  char *slice = WPW > 1 ? lds + w.win_bytes + wib * win_slice_bytes<P, U>() : xs + w.win_bytes;
   const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
   const X *xk = static_cast<const X *>(a.x) + j0;
-  Acc acc[KT];
+  Acc acc[KW];
   // window columns first, then the first value batches, then the x gathers
-  WinFill<X, KT> f;
-  f.cols(w.cols + w0, w1 - w0, lane);
+  WinFill<X, KT, 64 * WPW> f;
+  const int32_t first = WPW > 1 ? wib * 64 + lane : lane;
+  f.cols(w.cols + w0, w1 - w0, first);
   if constexpr (D == 1) {
+    static_assert(WPW == 1, "one batch in flight only with a window per wave");
     WinBatch<P, U> bt;
     const bool early = m.len >= (uint32_t)U;
     if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, lane);
-    window_group<P, U, KT>(acc, vbase, ibase, xs, m.len, bt, early, xs + w.win_bytes, lane);
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first);
+    window_group<P, U, KT>(acc, vbase, ibase, xs, m.len, bt, early, slice, lane);
   } else {
     WinPipe<P, U, D> pp;
     pp.prologue(vbase, ibase, m.len, lane);  // in flight while the window fills
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, lane);
-    window_group_deep<P, U, KT, D>(acc, vbase, ibase, xs, m.len, pp, xs + w.win_bytes, lane);
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first);
+    if constexpr (WPW > 1) __syncthreads();  // the window is the block's
+    window_group_deep<P, U, KT, KW, D>(acc, vbase, ibase, xs, m.len, pp, slice, t0, lane);
   }
   if constexpr (SPLIT) {
     for (uint32_t o = 1; o < m.tpr; o <<= 1) {
 #pragma unroll
-      for (int t = 0; t < KT; t++) {
+      for (int t = 0; t < KW; t++) {
         Acc other = P::shfl_down(acc[t], (int)o, W);
         P::add(acc[t], other);
       }
@@ -303,13 +316,14 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
   if (((uint32_t)lane & (m.tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
   const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
+  const int64_t c0 = (int64_t)j0 + t0;
 #pragma unroll
-  for (int t = 0; t < KT; t++) {
+  for (int t = 0; t < KW; t++) {
     const double res = P::result(acc[t]);
     if constexpr (OUT == OUT_RESID)
-      a.y[dst * a.ldy + j0 + t] = a.b[dst * a.ldb + j0 + t] - res;
+      a.y[dst * a.ldy + c0 + t] = a.b[dst * a.ldb + c0 + t] - res;
     else
-      a.y[dst * a.ldy + j0 + t] = res;
+      a.y[dst * a.ldy + c0 + t] = res;
   }
 }
 
@@ -612,46 +626,62 @@ LdsLimits lds_limits() {
 using WinFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
                        int32_t, hipStream_t);
 
-template <class P, bool SPLIT, int OUT, int U, int KT, int D>
+template <class P, bool SPLIT, int OUT, int U, int KT, int D, int WPW>
 void launch_win(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
                 int32_t nwork, int32_t ktiles, int32_t remap, hipStream_t s) {
-  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U, KT, D>), grid, block, lds, s, a, w, nwork,
-                     ktiles, remap);
+  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U, KT, D, WPW>), grid, block, lds, s, a, w,
+                     nwork, ktiles, remap);
 }
 
 // k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8 and
-// two batches in flight (measured best of one, two and four)
+// two batches in flight (measured best of one, two and four), and 1, 2 or 4
+// waves sharing a window where each wave's share is whole 16-byte chunks
 template <class P, bool SPLIT, int OUT>
-WinFn pick_win_u(int u, int kt) {
+WinFn pick_win_u(int u, int kt, int wpw) {
+  constexpr int xsz = (int)sizeof(typename P::X);
   if (kt == 1) {
-    if (u == 8) return &launch_win<P, SPLIT, OUT, 8, 1, 1>;
-    if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1, 1>;
+    if (wpw != 1) return nullptr;
+    if (u == 8) return &launch_win<P, SPLIT, OUT, 8, 1, 1, 1>;
+    if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1, 1, 1>;
     return nullptr;
   }
   if (u != 8) return nullptr;
-  if (kt == 2) return &launch_win<P, SPLIT, OUT, 8, 2, 2>;
-  if (kt == 4) return &launch_win<P, SPLIT, OUT, 8, 4, 2>;
-  if (kt == 8) return &launch_win<P, SPLIT, OUT, 8, 8, 2>;
+  if (kt == 2) return wpw == 1 ? &launch_win<P, SPLIT, OUT, 8, 2, 2, 1> : nullptr;
+  if (kt == 4) {
+    if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 4, 2, 1>;
+    if constexpr (2 * xsz >= 16) {
+      if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 4, 2, 2>;
+    }
+    return nullptr;
+  }
+  if (kt == 8) {
+    if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 8, 2, 1>;
+    if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 8, 2, 2>;
+    if constexpr (2 * xsz >= 16) {
+      if (wpw == 4) return &launch_win<P, SPLIT, OUT, 8, 8, 2, 4>;
+    }
+    return nullptr;
+  }
   return nullptr;
 }
 
 template <class P, bool SPLIT>
-WinFn pick_win_out(int out, int u, int kt) {
-  if (out == OUT_RUN) return pick_win_u<P, SPLIT, OUT_RUN>(u, kt);
-  if (out == OUT_RUNX) return pick_win_u<P, SPLIT, OUT_RUNX>(u, kt);
-  if constexpr (std::is_same_v<P, PF64>) return pick_win_u<P, SPLIT, OUT_RESID>(u, kt);
+WinFn pick_win_out(int out, int u, int kt, int wpw) {
+  if (out == OUT_RUN) return pick_win_u<P, SPLIT, OUT_RUN>(u, kt, wpw);
+  if (out == OUT_RUNX) return pick_win_u<P, SPLIT, OUT_RUNX>(u, kt, wpw);
+  if constexpr (std::is_same_v<P, PF64>) return pick_win_u<P, SPLIT, OUT_RESID>(u, kt, wpw);
   return nullptr;
 }
 
 template <class P>
-WinFn pick_win_split(bool split, int out, int u, int kt) {
-  return split ? pick_win_out<P, true>(out, u, kt) : pick_win_out<P, false>(out, u, kt);
+WinFn pick_win_split(bool split, int out, int u, int kt, int wpw) {
+  return split ? pick_win_out<P, true>(out, u, kt, wpw) : pick_win_out<P, false>(out, u, kt, wpw);
 }
 
-WinFn pick_win(int prec, bool split, int out, int u, int kt) {
-  if (prec == 0) return pick_win_split<PF64>(split, out, u, kt);
-  if (prec == 1) return pick_win_split<PF32C>(split, out, u, kt);
-  return pick_win_split<PF32D>(split, out, u, kt);
+WinFn pick_win(int prec, bool split, int out, int u, int kt, int wpw) {
+  if (prec == 0) return pick_win_split<PF64>(split, out, u, kt, wpw);
+  if (prec == 1) return pick_win_split<PF32C>(split, out, u, kt, wpw);
+  return pick_win_split<PF32D>(split, out, u, kt, wpw);
 }
 
 size_t staged_lds_per_wave(int prec, int u) {
@@ -722,14 +752,21 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
                           cfg.nontemporal &&
                           (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
     if (windowed) {
-      // per wave: the window (win_max x kt entries, 16-byte multiple), then a
-      // U-step slice of values and 16-bit positions
+      // k-tile 1: per wave, the window (win_max x kt entries, 16-byte
+      // multiple), then a U-step slice of values and 16-bit positions, one
+      // group per wave.  Wider k-tiles: the block's waves share one group's
+      // window, each with its own slice.
       const int u = kt == 1 ? cfg.unroll : 8;
+      // waves per window: waves_per_block, at most one per 16-byte chunk
+      const int chunks = std::max(1, kt * (int)xsz / 16);
+      const int wpw = kt == 1 ? 1 : std::min(wpb, chunks);
+      const int bw = wpw > 1 ? wpw : wpb;  // waves per block
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
       const size_t win_bytes = ((size_t)D.win_max * kt * xsz + 15) / 16 * 16;
-      const size_t wave_bytes = win_bytes + (size_t)u * 64 * (vbytes + sizeof(uint16_t));
-      const size_t need = wave_bytes * (size_t)wpb;
-      WinFn wf = pick_win(prec, split, out, u, kt);
+      const size_t slice_bytes = (size_t)u * 64 * (vbytes + sizeof(uint16_t));
+      const size_t wave_bytes = win_bytes + slice_bytes;
+      const size_t need = wpw > 1 ? win_bytes + slice_bytes * (size_t)wpw : wave_bytes * (size_t)wpb;
+      WinFn wf = pick_win(prec, split, out, u, kt, wpw);
       if (wf && need <= (size_t)lds_limits().per_block) {
         WinArgs w{};
         w.idx = D.win_idx;
@@ -737,8 +774,9 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         w.cols = (out == OUT_RUNX && D.win_cols_kr) ? D.win_cols_kr : D.win_cols;
         w.wave_bytes = (int32_t)wave_bytes;
         w.win_bytes = (int32_t)win_bytes;
-        const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
-        const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+        const int gpb = wpw > 1 ? 1 : wpb;  // groups per block
+        const int64_t nb = ((int64_t)D.ngroups + gpb - 1) / gpb;
+        const int64_t nf = ((int64_t)D.nrows - D.nznrows + bw * 64 - 1) / (bw * 64);
         const int64_t nwork = nb * ntiles;
         if (nwork + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
         a.x = static_cast<const char *>(x) + (size_t)j0 * xsz;
@@ -746,7 +784,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         a.b = b ? b + j0 : nullptr;
         a.k = ntiles * kt;
         if (nwork + nf > 0)
-          wf(a, w, dim3((uint32_t)(nwork + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nwork,
+          wf(a, w, dim3((uint32_t)(nwork + nf)), dim3(bw * 64), std::max(need, lds), (int32_t)nwork,
              ntiles, cfg.xcd_remap ? 1 : 0, stream);
         hip_check(hipGetLastError(), "spmv launch");
         j0 += ntiles * kt;

@@ -352,6 +352,24 @@ struct WinEntry {
       }
     }
   }
+  // values [t0, t0 + KW) of entry j, where KW values fill whole 16-byte
+  // chunks and t0 is a multiple of KW (KW = KT: the whole entry)
+  template <int KW>
+  __device__ static __forceinline__ void read_part(const char *xs, uint32_t j, uint32_t t0,
+                                                   X (&v)[KW]) {
+    if constexpr (KW == KT) {
+      read(xs, j, v);
+    } else {
+      static_assert(KW * sizeof(X) % 16 == 0 && kChunks > 1, "whole chunks of a chunked entry");
+      constexpr uint32_t nq = KW * sizeof(X) / 16;
+      const uint32_t base = j * kBytes, s = swz(j), q0 = t0 * (uint32_t)sizeof(X) / 16;
+#pragma unroll
+      for (uint32_t q = 0; q < nq; q++) {
+        const u32x4 c = *reinterpret_cast<const u32x4 *>(xs + base + 16u * ((q0 + q) ^ s));
+        __builtin_memcpy(&v[q * kPer], &c, 16);
+      }
+    }
+  }
   __device__ static __forceinline__ void write(char *xs, uint32_t j, const X (&v)[KT]) {
     if constexpr (kChunks == 1) {
       X *e = reinterpret_cast<X *>(xs + j * kBytes);
@@ -371,24 +389,28 @@ struct WinEntry {
 
 // The window fill in two halves, so that the column loads go out before a
 // group's first value batches and the x gathers right after them: cols()
-// loads the first C*64 window columns (C per lane), gather() reads x at them,
-// writes those entries, and repeats for any further columns.  C keeps the
-// gathered values of one round within about 96 registers.
-template <class X, int KT>
+// loads the first C*S window columns (C per lane, entries first + q*S),
+// gather() reads x at them, writes those entries, and repeats for any further
+// columns.  S is 64 times the number of waves that fill one window together.
+// C keeps one round's gathered values within about 96 registers, and one
+// round covers 768 entries (hex elasticity windows hold 646 on average).
+template <class X, int KT, int S = 64>
 struct WinFill {
   static constexpr int kRegs = KT * (int)sizeof(X) / 4;  // registers per entry
-  static constexpr int C = 96 / kRegs > 12 ? 12 : (96 / kRegs < 4 ? 4 : 96 / kRegs);
+  static constexpr int kByRegs = 96 / kRegs > 12 ? 12 : (96 / kRegs < 4 ? 4 : 96 / kRegs);
+  static constexpr int kByCover = 768 / S > 1 ? 768 / S : 1;
+  static constexpr int C = kByRegs < kByCover ? kByRegs : kByCover;
   int32_t c[C];
   __device__ __forceinline__ void cols(const int32_t *__restrict__ wcols, uint32_t n,
-                                       int32_t lane) {
+                                       int32_t first) {
 #pragma unroll
     for (int q = 0; q < C; q++) {
-      const uint32_t j = q * 64 + lane;
+      const uint32_t j = (uint32_t)first + q * S;
       c[q] = j < n ? wcols[j] : 0;
     }
   }
   __device__ __forceinline__ void gather(char *xs, const int32_t *__restrict__ wcols, uint32_t n,
-                                         const X *__restrict__ xb, int64_t ldx, int32_t lane) {
+                                         const X *__restrict__ xb, int64_t ldx, int32_t first) {
     for (uint32_t j0 = 0;;) {
       X v[C][KT];
 #pragma unroll
@@ -397,14 +419,14 @@ struct WinFill {
         for (int t = 0; t < KT; t++) v[q][t] = xb[(int64_t)c[q] * ldx + t];
 #pragma unroll
       for (int q = 0; q < C; q++) {
-        const uint32_t j = j0 + q * 64 + lane;
+        const uint32_t j = j0 + (uint32_t)first + q * S;
         if (j < n) WinEntry<X, KT>::write(xs, j, v[q]);
       }
-      j0 += C * 64;
+      j0 += C * S;
       if (j0 >= n) break;
 #pragma unroll
       for (int q = 0; q < C; q++) {
-        const uint32_t j = j0 + q * 64 + lane;
+        const uint32_t j = j0 + (uint32_t)first + q * S;
         c[q] = j < n ? wcols[j] : 0;
       }
     }
@@ -485,10 +507,11 @@ __device__ __forceinline__ void window_group(typename P::Acc (&acc)[KT], const t
 }
 
 // The x reads and FMAs of one staged batch: slot u of the lane reads its
-// value and window position from the slice, then its KT x values.
-template <class P, int U, int KT>
-__device__ __forceinline__ void window_batch(typename P::Acc (&acc)[KT], const char *xs,
-                                             const char *slice, int32_t lane) {
+// value and window position from the slice, then its KW x values (values
+// t0 .. t0+KW-1 of the entry).
+template <class P, int U, int KT, int KW>
+__device__ __forceinline__ void window_batch(typename P::Acc (&acc)[KW], const char *xs,
+                                             const char *slice, uint32_t t0, int32_t lane) {
   using V = typename P::V;
   using X = typename P::X;
   const V *lv = reinterpret_cast<const V *>(slice);
@@ -502,10 +525,10 @@ __device__ __forceinline__ void window_batch(typename P::Acc (&acc)[KT], const c
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    X xv[KT];
-    WinEntry<X, KT>::read(xs, p[u], xv);
+    X xv[KW];
+    WinEntry<X, KT>::template read_part<KW>(xs, p[u], t0, xv);
 #pragma unroll
-    for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xv[t]);
+    for (int t = 0; t < KW; t++) P::step(acc[t], v[u], xv[t]);
   }
 }
 
@@ -528,16 +551,16 @@ struct WinPipe {
   }
 };
 
-template <class P, int U, int KT, int D>
-__device__ __forceinline__ void window_group_deep(typename P::Acc (&acc)[KT],
+template <class P, int U, int KT, int KW, int D>
+__device__ __forceinline__ void window_group_deep(typename P::Acc (&acc)[KW],
                                                   const typename P::V *vbase,
                                                   const uint16_t *ibase, const char *xs,
                                                   uint32_t len, WinPipe<P, U, D> &pp, char *slice,
-                                                  int32_t lane) {
+                                                  uint32_t t0, int32_t lane) {
   using V = typename P::V;
   using X = typename P::X;
 #pragma unroll
-  for (int t = 0; t < KT; t++) P::init(acc[t]);  // every slot by fma onto -0
+  for (int t = 0; t < KW; t++) P::init(acc[t]);  // every slot by fma onto -0
   const uint32_t nb = len / U;
   uint32_t b = 0;
   // steady state: the batch D ahead of each consumed one exists
@@ -546,7 +569,7 @@ __device__ __forceinline__ void window_group_deep(typename P::Acc (&acc)[KT],
     for (int d = 0; d < D; d++) {
       pp.bt[d].stage(slice, lane);
       pp.bt[d].issue(vbase, ibase, (b + D + d) * U, lane);
-      window_batch<P, U, KT>(acc, xs, slice, lane);
+      window_batch<P, U, KT, KW>(acc, xs, slice, t0, lane);
     }
   }
   // drain: fewer than 2D batches left
@@ -555,13 +578,13 @@ __device__ __forceinline__ void window_group_deep(typename P::Acc (&acc)[KT],
     if (b + d < nb) {
       pp.bt[d].stage(slice, lane);
       if (b + D + d < nb) pp.bt[d].issue(vbase, ibase, (b + D + d) * U, lane);
-      window_batch<P, U, KT>(acc, xs, slice, lane);
+      window_batch<P, U, KT, KW>(acc, xs, slice, t0, lane);
     }
 #pragma unroll
   for (int d = 0; d < D; d++)
     if (b + D + d < nb) {
       pp.bt[d].stage(slice, lane);
-      window_batch<P, U, KT>(acc, xs, slice, lane);
+      window_batch<P, U, KT, KW>(acc, xs, slice, t0, lane);
     }
   // the last, partial batch: direct loads, all issued before the first use
   const uint32_t i = nb * U;
@@ -577,10 +600,10 @@ __device__ __forceinline__ void window_group_deep(typename P::Acc (&acc)[KT],
 #pragma unroll
     for (int u = 0; u < U - 1; u++)
       if (i + u < len) {
-        X xv[KT];
-        WinEntry<X, KT>::read(xs, c[u], xv);
+        X xv[KW];
+        WinEntry<X, KT>::template read_part<KW>(xs, c[u], t0, xv);
 #pragma unroll
-        for (int t = 0; t < KT; t++) P::step(acc[t], v[u], xv[t]);
+        for (int t = 0; t < KW; t++) P::step(acc[t], v[u], xv[t]);
       }
   }
 }

@@ -106,13 +106,20 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     // all k-tiles in one launch, a group's tiles on one XCD (remap) so its
     // values and positions are read from HBM once.  Measured on hex
     // elasticity 32^3 and 64^3 against the direct kernel: FP64 k=4 15-30%,
-    // k=8 20%, k=32 30% faster with k-tile 8; FP32 is bound by the
-    // compensated arithmetic and does best with k-tile 4.
+    // k=8 20%, k=32 30% faster with k-tile 8 and one wave per window.  FP32
+    // is bound by its arithmetic: from k=8 two waves share each k-tile-8
+    // window, four right-hand sides each (k=32: 91 against 106 us with
+    // k-tile 4), and below, two groups per block.
     c.stage_lds = 3;
     c.unroll = 8;
-    c.k_tile = std::min(dtype == WK_F64 ? 8 : 4, pow2_at_most(k));
-    c.waves_per_block = 1;
     c.xcd_remap = 1;
+    if (dtype == WK_F64) {
+      c.k_tile = std::min(8, pow2_at_most(k));
+      c.waves_per_block = 1;
+    } else {
+      c.k_tile = k >= 8 ? 8 : pow2_at_most(k);
+      c.waves_per_block = 2;
+    }
   } else if (k == 1 && D.W == 64 && mean_len >= 20.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
     // (measured best on hex diffusion, 27-entry rows)
