@@ -27,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# unpacked FP32 VALU lane operations per second (MI355X: 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz; the published 157.3 TFLOP/s counts FMA and packed pairs)
+VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9
 METRIC = "SpMV GB/s & % HBM roofline, FP64 vs FP32-comp multi-RHS; fwd+adjoint ms"
 
 
@@ -160,8 +163,17 @@ def extra_multi_rhs(torch, dev, steps, warmup, tune=True):
         r32 = m.nnz * k / (ms32 * 1e-3)
         if k == 1:
             base = r64
+        # bounds: HBM at the nominal peak for the algorithmic bytes, and for the
+        # compensated form the FP32 VALU time of its ten unpacked operations
+        # per entry and right-hand side (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz)
+        hbm64 = spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (HBM_PEAK_GBS * 1e9) * 1e3
+        hbm32 = spmv_bytes(m.nnz, m.nrows, m.ncols, k, 4, 4, 8) / (HBM_PEAK_GBS * 1e9) * 1e3
+        valu32 = m.nnz * k * 10 / VALU_LANE_OPS_PER_S * 1e3
         rows.append({
             "k": k, "fp64_ms": round(ms64, 4), "fp32c_ms": round(ms32, 4), "fp32_f64acc_ms": round(ms32d, 4),
+            "fp64_hbm_bound_ms": round(hbm64, 4),
+            "fp32c_hbm_bound_ms": round(hbm32, 4), "fp32c_valu_bound_ms": round(valu32, 4),
+            "fp32c_frac_of_bound": round(max(hbm32, valu32) / ms32, 3),
             "fp64_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (ms64 * 1e-3) / 1e9, 1),
             "fp32c_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k, 4, 4, 8) / (ms32 * 1e-3) / 1e9, 1),
             "fp32c_nnz_rhs_per_s_vs_fp64_k1": round(r32 / base, 3),
