@@ -1,6 +1,7 @@
 // spmv_device.hpp -- device code shared by the SpMV kernels (kernels.hip)
 // and the batched persistent kernel (batch.hip): the per-precision
-// arithmetic, the direct batch loops and the LDS-staged group loop.
+// arithmetic, the direct batch loops, the LDS-staged group loop and the x
+// window (entry layout, fill, one- and two-batch-deep group loops).
 //
 // Summation order per lane is the reference's (warpkernel.hpp:68-71): the
 // rounded first product, then one fma per further slot, trailing pads
@@ -287,13 +288,9 @@ __host__ __device__ constexpr int win_slice_bytes() {
   return U * 64 * (int)(sizeof(typename P::V) + sizeof(uint16_t));
 }
 
-// One group through its x window.  xs holds x at the group's distinct columns
-// (filled by fill_window); each slot carries the 16-bit position of its column
-// there.  Values and positions stream in U-step batches with 16-byte loads
-// through the slice, the next batch in flight during the current FMAs, and x
-// comes from LDS: the same products in the same order as staged_group, without
-// a column index or an x gather per slot.  The caller has issued the first
-// batch (tv/ti) before filling the window.
+// One U-step batch of a group's values and 16-bit window positions: issue()
+// loads it with 16-byte loads into registers, stage() copies it into the
+// wave's LDS slice, from which each lane reads its own slots.
 template <class P, int U>
 struct WinBatch {
   static constexpr int NV = U * 64 * (int)sizeof(typename P::V) / 1024;
@@ -442,6 +439,13 @@ __device__ __forceinline__ void fill_window(char *xs, const int32_t *__restrict_
   f.gather(xs, wcols, n, xb, ldx, lane);
 }
 
+// One group through its x window.  xs holds x at the group's distinct columns
+// (filled by fill_window); each slot carries the 16-bit position of its column
+// there.  Values and positions stream in U-step batches through the slice,
+// the next batch in flight during the current FMAs, and x comes from LDS: the
+// same products in the same order as staged_group, without a column index or
+// an x gather per slot.  When `issued`, the caller has issued the first batch
+// into bt before filling the window.
 template <class P, int U, int KT>
 __device__ __forceinline__ void window_group(typename P::Acc (&acc)[KT], const typename P::V *vbase,
                                              const uint16_t *ibase, const char *xs, uint32_t len,
