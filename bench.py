@@ -211,9 +211,13 @@ def extra_fwd_adjoint(torch, dev, steps, warmup, tune=True):
         # one training step's SpMV work back to back: forward, then backward
         _, seq = time_launches(torch, lambda: (eng.launch(x, y), eng.backward(dyv, x, dxv, dv)),
                                steps, warmup)
+        # the same step with dx and dvals as two launches (for comparison with
+        # wk_backward's choice)
+        _, seq2 = time_launches(torch, lambda: (eng.launch(x, y), eng.run_adjoint(dyv, dxv),
+                                                eng.dvals(dyv, x, dv)), steps, warmup)
         fa[f"k{k}"] = {"fwd_ms": round(f, 4), "dx_ms": round(a, 4), "dvals_ms": round(d, 4),
                        "backward_ms": round(bw, 4), "fwd_adjoint_ms": round(seq, 4),
-                       "fwd_plus_separate_dx_dvals_ms": round(f + a + d, 4),
+                       "fwd_adjoint_two_launch_backward_ms": round(seq2, 4),
                        "fwd_GBs": round(spmv_bytes(m.nnz, m.nrows, m.ncols, k) / (f * 1e-3) / 1e9, 1)}
     out["fwd_adjoint_hex_laplace_3d_48"] = fa
     del eng
