@@ -24,6 +24,18 @@ def test_tune_layout_returns_a_working_engine():
     y = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
     best = tune_layout(m.rowptr, m.colind, m.vals, x, y, reps=3, threshold_list=[4, 7, 14])
     assert best.variant in (1, 2) and best.ms > 0 and len(best.table) == 4
+    _check_choice(best, m, x, y)
+    # group widths 64 and 32, lanes-per-row caps 32 and 64 (64 only at width 64)
+    best = tune_layout(m.rowptr, m.colind, m.vals, x, y, reps=2, threshold_list=[4, 14],
+                       group_width=(64, 32), max_tpr=(32, 64))
+    assert len(best.table) == 1 + 2 + 2 + 1 + 2
+    assert {(t[3], t[4]) for t in best.table if t[0] == 2} == {(64, 32), (64, 64), (32, 32)}
+    _check_choice(best, m, x, y)
+
+
+def _check_choice(best, m, x, y):
+    import torch
+    from oracle import pyoracle as O
     y.fill_(float("nan"))
     best.engine.launch(x, y)
     torch.cuda.synchronize()
@@ -31,9 +43,10 @@ def test_tune_layout_returns_a_working_engine():
     if best.variant == 1:
         assert np.array_equal(y.cpu().numpy(), ref)
     else:
-        L = O.scan2(m.rowptr, best.threshold, 64, 32)
+        L = O.scan2(m.rowptr, best.threshold, best.group_width, best.max_tpr)
         pv, pc = O.process(L, m.vals, m.colind)
         assert np.array_equal(y.cpu().numpy(), O.wpk_spmv(L, pv, pc, x.cpu().numpy()))
+    best.engine.close()
 
 
 @pytest.mark.gpu

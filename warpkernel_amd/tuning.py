@@ -1,4 +1,5 @@
-"""Layout-level tuning: WPK1 against WPK2 over a threshold sweep.
+"""Layout-level tuning: WPK1 against WPK2 over a threshold sweep, optionally
+over group widths and WPK2's lanes-per-row cap too.
 
 The reference picks its layout by sweeping structure2's threshold from min_nz to
 max_nz in steps of 10% of the range, and warps/block 2..16, keeping the fastest
@@ -23,7 +24,10 @@ class LayoutChoice:
     launch: dict
     layout: object = field(repr=False)
     engine: object = field(repr=False)
-    table: list = field(default_factory=list, repr=False)  # (variant, threshold, ms) of every candidate
+    # every candidate: (variant, threshold, ms, group width, max lanes per row)
+    table: list = field(default_factory=list, repr=False)
+    group_width: int = 64
+    max_tpr: int = 32
 
 
 def thresholds(min_nz: int, max_nz: int) -> list[int]:
@@ -32,10 +36,16 @@ def thresholds(min_nz: int, max_nz: int) -> list[int]:
     return list(range(int(min_nz), int(max_nz) + 1, max(1, inc)))
 
 
-def tune_layout(rowptr, colind, vals, x, y, ncols: int | None = None, group_width: int = 64,
-                max_tpr: int = 32, reps: int = 10, threshold_list=None, dtype: str = "f64",
+def _as_tuple(v):
+    return tuple(v) if isinstance(v, (list, tuple)) else (v,)
+
+
+def tune_layout(rowptr, colind, vals, x, y, ncols: int | None = None, group_width=64,
+                max_tpr=32, reps: int = 10, threshold_list=None, dtype: str = "f64",
                 kr: bool = False, adjoint: bool = False) -> LayoutChoice:
     """x, y: device tensors of the operator's shape (k right-hand sides allowed).
+    group_width and max_tpr may be single values or sequences (e.g. (64, 32) and
+    (32, 64)): every combination is a candidate, max_tpr at most the width.
     Returns the fastest layout with its engine (other engines are released)."""
     rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
     nrows = rowptr.size - 1
@@ -43,24 +53,28 @@ def tune_layout(rowptr, colind, vals, x, y, ncols: int | None = None, group_widt
     best = None
     table = []
 
-    def consider(variant, th, s):
+    def consider(variant, th, s, w, tpr):
         nonlocal best
         eng = engine(s, vals, colind, dtype=dtype, kr=kr, adjoint=adjoint)
         cfg, ms = eng.tune(x, y, reps=reps)
-        table.append((variant, th, ms))
+        table.append((variant, th, ms, w, tpr))
         if best is None or ms < best.ms:
-            best = LayoutChoice(variant, th, ms, cfg, s, eng)
+            best = LayoutChoice(variant, th, ms, cfg, s, eng, group_width=w, max_tpr=tpr)
         else:
             eng.close()
 
-    consider(1, None, structure(group_width).scan(nnz, nrows, vals, rowptr, colind, ncols=ncols))
-    probe = structure2(group_width, max_tpr).scan(nnz, nrows, vals, rowptr, colind, max(nrows, 1),
-                                                  ncols=ncols)
-    ths = threshold_list if threshold_list is not None else thresholds(probe.min_nz, probe.max_nz)
-    for th in ths:
-        if th >= probe.max_nz:
-            continue  # no group is split: the WPK1 layout again
-        s2 = structure2(group_width, max_tpr).scan(nnz, nrows, vals, rowptr, colind, int(th), ncols=ncols)
-        consider(2, int(th), s2)
+    for w in _as_tuple(group_width):
+        consider(1, None, structure(w).scan(nnz, nrows, vals, rowptr, colind, ncols=ncols), w, None)
+        for tpr in _as_tuple(max_tpr):
+            if tpr > w:
+                continue
+            probe = structure2(w, tpr).scan(nnz, nrows, vals, rowptr, colind, max(nrows, 1),
+                                            ncols=ncols)
+            ths = threshold_list if threshold_list is not None else thresholds(probe.min_nz, probe.max_nz)
+            for th in ths:
+                if th >= probe.max_nz:
+                    continue  # no group is split: the WPK1 layout again
+                s2 = structure2(w, tpr).scan(nnz, nrows, vals, rowptr, colind, int(th), ncols=ncols)
+                consider(2, int(th), s2, w, tpr)
     best.table = table
     return best
