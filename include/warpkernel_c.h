@@ -256,7 +256,9 @@ typedef struct wk_launch {
                               group width 64, rows_per_lane 1, nontemporal 1,
                               and k_tile 1 with unroll 8/16; stage 3 also to
                               k_tiles 2, 4 and 8 (unroll 8), all k-tiles in
-                              one launch; otherwise ignored */
+                              one launch; 4: as 3, with values and window
+                              positions loaded straight into registers
+                              instead of through LDS; otherwise ignored */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);

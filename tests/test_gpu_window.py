@@ -162,7 +162,8 @@ def test_multi_rhs_all_paths_bitexact(k):
     m = fe.CONFIGS["hex_elastic_3d_32"]() if k in (4, 8) else fe.tet_poisson_3d(40, drop=0.05)
     x = fe.uniform_x(m.ncols, k, seed=4000 + k).reshape(m.ncols, k)
     cfgs = [None] + [dict(k_tile=t, stage_lds=0) for t in (1, 2, 4, 8, 16, 32) if t <= k] + \
-        [dict(WIN, k_tile=t, unroll=8, waves_per_block=w) for t in (2, 4, 8) if t <= k for w in (1, 2, 4)]
+        [dict(WIN, k_tile=t, unroll=8, waves_per_block=w, stage_lds=st) for t in (2, 4, 8) if t <= k
+         for w in (1, 2, 4) for st in (3, 4)]
     for variant in (1, 2):
         if variant == 1:
             s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
@@ -198,9 +199,11 @@ def test_multi_rhs_fp32_windowed_matches_direct(k):
         y1 = torch.full_like(y0, float("nan"))
         eng.launch(x, y0, cfg=dict(k_tile=k, stage_lds=0, accumulate=acc))
         for wpb in (1, 2):  # k-tile 8: two waves share each window
-            y1.fill_(float("nan"))
-            eng.launch(x, y1, cfg=dict(WIN, k_tile=k, unroll=8, waves_per_block=wpb, accumulate=acc))
-            assert torch.equal(y0, y1), (acc, wpb)
+            for st in (3, 4):
+                y1.fill_(float("nan"))
+                eng.launch(x, y1, cfg=dict(WIN, k_tile=k, unroll=8, waves_per_block=wpb, accumulate=acc,
+                                           stage_lds=st))
+                assert torch.equal(y0, y1), (acc, wpb, st)
     eng.close()
 
 
@@ -220,7 +223,8 @@ def test_multi_rhs_residual_and_strided_windows(k):
     eng = engine(s, m.vals, m.colind)
     xd, bd = dev(xw), dev(bw)
     for cfg in (None, dict(WIN, k_tile=2, unroll=8, waves_per_block=1, xcd_remap=1),
-                dict(WIN, k_tile=4, unroll=8, waves_per_block=2, xcd_remap=0)):
+                dict(WIN, k_tile=4, unroll=8, waves_per_block=2, xcd_remap=0),
+                dict(WIN, k_tile=8, unroll=8, waves_per_block=2, xcd_remap=1, stage_lds=4)):
         yw = torch.full((m.nrows, ld), float("nan"), dtype=torch.float64, device="cuda")
         eng.launch(xd[:, :k], yw[:, :k], cfg=cfg)
         got = host(yw)

@@ -248,7 +248,9 @@ struct WinArgs {
 // (k-tiles above 1): the block's WPW waves take one group, fill its window
 // together (one barrier), and wave w sums values [w*KT/WPW, (w+1)*KT/WPW) of
 // the k-tile, so a window serves WPW waves.  LDS: per wave [window | slice]
-// (WPW = 1), or [window | slice of wave 0 | ... ] (WPW > 1).
+// (WPW = 1), or [window | slice of wave 0 | ... ] (WPW > 1).  D batches of
+// values and positions in flight through the slices, or D = 0: loaded
+// straight into registers, no slices.
 template <class P, bool SPLIT, int OUT, int U, int KT, int D, int WPW>
 __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
                                                                int32_t ktiles, int32_t remap) {
@@ -283,6 +285,7 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   const uint16_t *ibase = w.idx + m.base;
   char *xs = WPW > 1 ? lds : lds + wib * w.wave_bytes;
   char *slice = WPW > 1 ? lds + w.win_bytes + wib * win_slice_bytes<P, U>() : xs + w.win_bytes;
+  (void)slice;  // D = 0: no slice
   const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
   const X *xk = static_cast<const X *>(a.x) + j0;
   Acc acc[KW];
@@ -290,7 +293,12 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   WinFill<X, KT, 64 * WPW> f;
   const int32_t first = WPW > 1 ? wib * 64 + lane : lane;
   f.cols(w.cols + w0, w1 - w0, first);
-  if constexpr (D == 1) {
+  if constexpr (D == 0) {
+    // values and positions straight into registers (no slice)
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first);
+    if constexpr (WPW > 1) __syncthreads();  // the window is the block's
+    window_group_regs<P, U, KT, KW>(acc, vbase, ibase, xs, m.len, t0, lane);
+  } else if constexpr (D == 1) {
     static_assert(WPW == 1, "one batch in flight only with a window per wave");
     WinBatch<P, U> bt;
     const bool early = m.len >= (uint32_t)U;
@@ -636,52 +644,63 @@ void launch_win(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size
 // k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8 and
 // two batches in flight (measured best of one, two and four), and 1, 2 or 4
 // waves sharing a window where each wave's share is whole 16-byte chunks
-template <class P, bool SPLIT, int OUT>
-WinFn pick_win_u(int u, int kt, int wpw) {
+template <class P, bool SPLIT, int OUT, int D>
+WinFn pick_win_d(int u, int kt, int wpw) {
   constexpr int xsz = (int)sizeof(typename P::X);
+  constexpr int D1 = D == 0 ? 0 : 1;  // single right-hand side: one batch in flight
   if (kt == 1) {
     if (wpw != 1) return nullptr;
-    if (u == 8) return &launch_win<P, SPLIT, OUT, 8, 1, 1, 1>;
-    if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1, 1, 1>;
+    if (u == 8) return &launch_win<P, SPLIT, OUT, 8, 1, D1, 1>;
+    if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1, D1, 1>;
     return nullptr;
   }
   if (u != 8) return nullptr;
-  if (kt == 2) return wpw == 1 ? &launch_win<P, SPLIT, OUT, 8, 2, 2, 1> : nullptr;
+  if (kt == 2) return wpw == 1 ? &launch_win<P, SPLIT, OUT, 8, 2, D, 1> : nullptr;
   if (kt == 4) {
-    if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 4, 2, 1>;
+    if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 4, D, 1>;
     if constexpr (2 * xsz >= 16) {
-      if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 4, 2, 2>;
+      if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 4, D, 2>;
     }
     return nullptr;
   }
   if (kt == 8) {
-    if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 8, 2, 1>;
-    if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 8, 2, 2>;
+    if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 8, D, 1>;
+    if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 8, D, 2>;
     if constexpr (2 * xsz >= 16) {
-      if (wpw == 4) return &launch_win<P, SPLIT, OUT, 8, 8, 2, 4>;
+      if (wpw == 4) return &launch_win<P, SPLIT, OUT, 8, 8, D, 4>;
     }
     return nullptr;
   }
   return nullptr;
 }
 
+// k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8 and
+// two batches in flight through the slices (measured best of one, two and
+// four) or, regs, loaded straight into registers; 1, 2 or 4 waves sharing a
+// window where each wave's share is whole 16-byte chunks
+template <class P, bool SPLIT, int OUT>
+WinFn pick_win_u(int u, int kt, int wpw, bool regs) {
+  return regs ? pick_win_d<P, SPLIT, OUT, 0>(u, kt, wpw) : pick_win_d<P, SPLIT, OUT, 2>(u, kt, wpw);
+}
+
 template <class P, bool SPLIT>
-WinFn pick_win_out(int out, int u, int kt, int wpw) {
-  if (out == OUT_RUN) return pick_win_u<P, SPLIT, OUT_RUN>(u, kt, wpw);
-  if (out == OUT_RUNX) return pick_win_u<P, SPLIT, OUT_RUNX>(u, kt, wpw);
-  if constexpr (std::is_same_v<P, PF64>) return pick_win_u<P, SPLIT, OUT_RESID>(u, kt, wpw);
+WinFn pick_win_out(int out, int u, int kt, int wpw, bool regs) {
+  if (out == OUT_RUN) return pick_win_u<P, SPLIT, OUT_RUN>(u, kt, wpw, regs);
+  if (out == OUT_RUNX) return pick_win_u<P, SPLIT, OUT_RUNX>(u, kt, wpw, regs);
+  if constexpr (std::is_same_v<P, PF64>) return pick_win_u<P, SPLIT, OUT_RESID>(u, kt, wpw, regs);
   return nullptr;
 }
 
 template <class P>
-WinFn pick_win_split(bool split, int out, int u, int kt, int wpw) {
-  return split ? pick_win_out<P, true>(out, u, kt, wpw) : pick_win_out<P, false>(out, u, kt, wpw);
+WinFn pick_win_split(bool split, int out, int u, int kt, int wpw, bool regs) {
+  return split ? pick_win_out<P, true>(out, u, kt, wpw, regs)
+               : pick_win_out<P, false>(out, u, kt, wpw, regs);
 }
 
-WinFn pick_win(int prec, bool split, int out, int u, int kt, int wpw) {
-  if (prec == 0) return pick_win_split<PF64>(split, out, u, kt, wpw);
-  if (prec == 1) return pick_win_split<PF32C>(split, out, u, kt, wpw);
-  return pick_win_split<PF32D>(split, out, u, kt, wpw);
+WinFn pick_win(int prec, bool split, int out, int u, int kt, int wpw, bool regs) {
+  if (prec == 0) return pick_win_split<PF64>(split, out, u, kt, wpw, regs);
+  if (prec == 1) return pick_win_split<PF32C>(split, out, u, kt, wpw, regs);
+  return pick_win_split<PF32D>(split, out, u, kt, wpw, regs);
 }
 
 size_t staged_lds_per_wave(int prec, int u) {
@@ -700,7 +719,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
                  int64_t ldx, double *y, int64_t ldy, const double *b, int64_t ldb, int32_t k,
                  hipStream_t stream) {
   if (k < 1) fail(WK_ERR_ARG, "k must be >= 1");
-  if (cfg.stage_lds < 0 || cfg.stage_lds > 3) fail(WK_ERR_ARG, "stage_lds must be 0, 1, 2 or 3");
+  if (cfg.stage_lds < 0 || cfg.stage_lds > 4) fail(WK_ERR_ARG, "stage_lds must be 0, 1, 2, 3 or 4");
   const int wpb = cfg.waves_per_block;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) fail(WK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
   if (cfg.k_tile < 1 || cfg.k_tile > 32 || (cfg.k_tile & (cfg.k_tile - 1)))
@@ -748,7 +767,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const bool staged = cfg.stage_lds && kt == 1 && D.W == 64 && r == 1 && cfg.nontemporal &&
                         (cfg.unroll == 8 || cfg.unroll == 16);
     // x windows: any k-tile up to 8 (unroll 8 for k-tiles above 1)
-    const bool windowed = cfg.stage_lds == 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
+    const bool windowed = cfg.stage_lds >= 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
                           cfg.nontemporal &&
                           (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
     if (windowed) {
@@ -763,10 +782,11 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       const int bw = wpw > 1 ? wpw : wpb;  // waves per block
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
       const size_t win_bytes = ((size_t)D.win_max * kt * xsz + 15) / 16 * 16;
-      const size_t slice_bytes = (size_t)u * 64 * (vbytes + sizeof(uint16_t));
+      const bool regs = cfg.stage_lds == 4;  // no slices
+      const size_t slice_bytes = regs ? 0 : (size_t)u * 64 * (vbytes + sizeof(uint16_t));
       const size_t wave_bytes = win_bytes + slice_bytes;
       const size_t need = wpw > 1 ? win_bytes + slice_bytes * (size_t)wpw : wave_bytes * (size_t)wpb;
-      WinFn wf = pick_win(prec, split, out, u, kt, wpw);
+      WinFn wf = pick_win(prec, split, out, u, kt, wpw, regs);
       if (wf && need <= (size_t)lds_limits().per_block) {
         WinArgs w{};
         w.idx = D.win_idx;

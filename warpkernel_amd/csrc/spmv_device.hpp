@@ -555,6 +555,81 @@ struct WinPipe {
   }
 };
 
+// window_group for wide k-tiles without the slice: each lane loads its own
+// values and window positions (8 + 2 bytes per slot, narrow loads) straight
+// into registers, two batches alternating so that one loads while the other
+// is summed.  With KT values per slot the x reads dominate the LDS traffic,
+// and staging the values through LDS would add to it.  Same products, same
+// order.
+template <class P, int U, int KT, int KW>
+__device__ __forceinline__ void window_group_regs(typename P::Acc (&acc)[KW],
+                                                  const typename P::V *vbase,
+                                                  const uint16_t *ibase, const char *xs,
+                                                  uint32_t len, uint32_t t0, int32_t lane) {
+  using V = typename P::V;
+  using X = typename P::X;
+#pragma unroll
+  for (int t = 0; t < KW; t++) P::init(acc[t]);  // every slot by fma onto -0
+  const V *vp = vbase + lane;
+  const uint16_t *ip = ibase + lane;
+  auto load = [&](uint32_t at, V (&v)[U], uint16_t (&c)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      c[u] = ip[(int64_t)(at + u) * 64];
+      v[u] = __builtin_nontemporal_load(vp + (int64_t)(at + u) * 64);
+    }
+  };
+  auto sum = [&](const V (&v)[U], const uint16_t (&c)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      X xv[KW];
+      WinEntry<X, KT>::template read_part<KW>(xs, c[u], t0, xv);
+#pragma unroll
+      for (int t = 0; t < KW; t++) P::step(acc[t], v[u], xv[t]);
+    }
+  };
+  const uint32_t nb = len / U;
+  uint32_t b = 0;
+  V va[U], vb[U];
+  uint16_t ca[U], cb[U];
+  if (nb > 0) load(0, va, ca);
+  // two batches per pass: B loads while A is summed, then A while B is
+  for (; b + 3 <= nb; b += 2) {
+    load((b + 1) * U, vb, cb);
+    sum(va, ca);
+    load((b + 2) * U, va, ca);
+    sum(vb, cb);
+  }
+  // one or two full batches left (A holds batch b)
+  if (b + 2 == nb) {
+    load((b + 1) * U, vb, cb);
+    sum(va, ca);
+    sum(vb, cb);
+  } else if (b + 1 == nb) {
+    sum(va, ca);
+  }
+  // the last, partial batch
+  const uint32_t i = nb * U;
+  if (i < len) {
+    V v[U];
+    uint16_t c[U];
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) {
+        c[u] = ip[(int64_t)(i + u) * 64];
+        v[u] = __builtin_nontemporal_load(vp + (int64_t)(i + u) * 64);
+      }
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) {
+        X xv[KW];
+        WinEntry<X, KT>::template read_part<KW>(xs, c[u], t0, xv);
+#pragma unroll
+        for (int t = 0; t < KW; t++) P::step(acc[t], v[u], xv[t]);
+      }
+  }
+}
+
 template <class P, int U, int KT, int KW, int D>
 __device__ __forceinline__ void window_group_deep(typename P::Acc (&acc)[KW],
                                                   const typename P::V *vbase,

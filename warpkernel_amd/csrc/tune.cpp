@@ -68,8 +68,12 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
           c.k_tile = kt;
           out.push_back(c);
           if (D.win_max > 0 && kt >= 2 && kt <= 8) {
-            c.stage_lds = 3;  // windowed pass (falls back where the window does not fit)
-            out.push_back(c);
+            // windowed passes, values through slices or straight into registers
+            // (the launcher falls back where the window does not fit)
+            for (int st : {3, 4}) {
+              c.stage_lds = st;
+              out.push_back(c);
+            }
             c.stage_lds = 0;
           }
         }
@@ -104,21 +108,39 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
   } else if (k >= 2 && windows) {
     // several right-hand sides through windows of k-tile values per column,
     // all k-tiles in one launch, a group's tiles on one XCD (remap) so its
-    // values and positions are read from HBM once.  Measured on hex
-    // elasticity 32^3 and 64^3 against the direct kernel: FP64 k=4 15-30%,
-    // k=8 20%, k=32 30% faster with k-tile 8 and one wave per window.  FP32
-    // is bound by its arithmetic: from k=8 two waves share each k-tile-8
-    // window, four right-hand sides each (k=32: 91 against 106 us with
-    // k-tile 4), and below, two groups per block.
-    c.stage_lds = 3;
+    // values and positions come from HBM once.  Up to k = 16 the values and
+    // positions go straight into registers (stage 4): the x reads dominate
+    // the LDS traffic, and staging would add to it.  Measured on hex
+    // elasticity 32^3 and 64^3: FP64 k=4 17.3 / 119 us (stage 3: 23.7 /
+    // 128), k=8 and 16 with k-tile 8 shared by two waves; at k=32 the
+    // staged form with one wave per window stays ahead (111 / 817 us).  FP32
+    // is bound by its arithmetic: k-tile 8 shared by two waves from k=8
+    // (k=32: 90 / 626 us), four windows per block below.
     c.unroll = 8;
     c.xcd_remap = 1;
     if (dtype == WK_F64) {
-      c.k_tile = std::min(8, pow2_at_most(k));
-      c.waves_per_block = 1;
+      if (k < 8) {
+        c.stage_lds = 4;
+        c.k_tile = pow2_at_most(k);
+        c.waves_per_block = 1;
+      } else if (k <= 16) {
+        c.stage_lds = 4;
+        c.k_tile = 8;
+        c.waves_per_block = 2;
+      } else {
+        c.stage_lds = 3;
+        c.k_tile = 8;
+        c.waves_per_block = 1;
+      }
     } else {
-      c.k_tile = k >= 8 ? 8 : pow2_at_most(k);
-      c.waves_per_block = 2;
+      c.stage_lds = 4;
+      if (k < 8) {
+        c.k_tile = pow2_at_most(k);
+        c.waves_per_block = 4;
+      } else {
+        c.k_tile = 8;
+        c.waves_per_block = 2;
+      }
     }
   } else if (k == 1 && D.W == 64 && mean_len >= 20.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
