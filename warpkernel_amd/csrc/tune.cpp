@@ -90,11 +90,18 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
   const double mean_len =
       D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * D.W) : 0.0;
   // x windows pay where each window entry serves enough slots to amortise its
-  // fill: hex elasticity (7.6 slots per entry for WPK1, 6.0 for WPK2 at
-  // threshold 27) gains 10-30%; hex diffusion (3.4), P1 tets (2.1) and the
-  // 2D Laplacian (2.9) lose
-  const bool windows = D.W == 64 && D.win_max > 0 && D.win_total > 0 &&
-                       (double)D.allocate_nz >= 4.5 * (double)D.win_total;
+  // fill: at k = 1, hex elasticity (7.6 slots per entry for WPK1, 6.0 for
+  // WPK2 at threshold 27) gains 10-30%, while hex diffusion (3.4), P1 tets
+  // (2.1) and the 2D Laplacian (2.9) lose.  With several right-hand sides
+  // each entry serves k_tile products per slot: hex diffusion gains 35-45%
+  // from k=2 (k=32: 128 -> 68 us), P1 tets from k=16 in FP64 (620 -> 476 us)
+  // and from k=2 in FP32; the tiny 2D Laplacian only at k=32.
+  const bool win_ok = D.W == 64 && D.win_max > 0 && D.win_total > 0;
+  const double reuse = win_ok ? (double)D.allocate_nz / (double)D.win_total : 0.0;
+  const bool windows =
+      win_ok && (k == 1 ? reuse >= 4.5
+                        : reuse >= 3.0 || (reuse >= 2.0 && (k >= 16 || (dtype != WK_F64 &&
+                                                                         D.ngroups >= 1024))));
   if (k == 1 && windows) {
     // 10 bytes per slot, x from the group's window in LDS; one wave per
     // block, a contiguous range of groups per XCD, and 16-step batches for
@@ -123,7 +130,9 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         c.stage_lds = 4;
         c.k_tile = pow2_at_most(k);
         c.waves_per_block = 1;
-      } else if (k <= 16) {
+      } else if (k <= 16 || mean_len < 40.0) {
+        // (short rows keep the register-fed form at every k: hex diffusion
+        // k=32 68 us against 86 staged)
         c.stage_lds = 4;
         c.k_tile = 8;
         c.waves_per_block = 2;
