@@ -37,11 +37,19 @@ def spmv_bytes(nnz, nrows, ncols, k=1, s_val=8, s_x=8, s_y=8):
     return nnz * (s_val + 4) + k * (ncols * s_x + nrows * s_y)
 
 
-def time_launches(torch, fn, steps, warmup):
+def time_launches(torch, fn, steps, warmup, min_warm_s=0.02):
     """Returns (wall seconds for `steps` launches, mean launch ms from HIP events
     recorded on the launch stream)."""
-    for _ in range(warmup):
+    # warm-up: `warmup` launches and at least min_warm_s of device time, so
+    # that short kernels are not timed while the clocks are still rising (the
+    # headline passes 0: exactly --warmup steps)
+    t_w = time.perf_counter()
+    n = 0
+    while n < warmup or time.perf_counter() - t_w < min_warm_s:
         fn()
+        n += 1
+        if n % 32 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -400,7 +408,7 @@ def main():
         print("[bench] autotuning launch shape", file=sys.stderr, flush=True)
         tuned, tuned_ms = eng.tune(dx, dy, reps=10)
     print("[bench] timing", file=sys.stderr, flush=True)
-    wall, ev_ms = time_launches(torch, lambda: eng.launch(dx, dy), args.steps, args.warmup)
+    wall, ev_ms = time_launches(torch, lambda: eng.launch(dx, dy), args.steps, args.warmup, 0.0)
     nbytes = spmv_bytes(m.nnz, m.nrows, m.ncols)
     value = nbytes * args.steps / wall / 1e9
     achieved = nbytes / (ev_ms * 1e-3) / 1e9

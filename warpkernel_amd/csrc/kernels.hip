@@ -393,6 +393,9 @@ struct DvArgs {
   double *dv;
 };
 
+// MULTI = false: one right-hand side only (the caller guarantees d.k == 1),
+// which keeps the k-chunk registers out of the fused backward kernel.
+template <bool MULTI>
 __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_t *srow) {
   constexpr int PER = kDvTile / kDvThreads;
   const int64_t e0 = (int64_t)tile * kDvTile;
@@ -411,7 +414,7 @@ __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_
     const int64_t b = d.rowptr[r + 1] < e1 ? d.rowptr[r + 1] : e1;
     for (int64_t e = a; e < b; e++) srow[e - e0] = r;
   }
-  if (d.k == 1) {
+  if (!MULTI || d.k == 1) {
     double xv[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) xv[q] = d.x[(int64_t)c[q] * d.ldx];
@@ -430,21 +433,48 @@ __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_
     return;
   }
   __syncthreads();
-#pragma unroll
+  // k right-hand sides: the first product, then one fma per further pair (by
+  // fma onto -0, as in the SpMV), 8 pairs at a time with all 16 loads of a
+  // chunk issued before its fmas
+  constexpr int KC = 8;
+#pragma unroll 2
   for (int q = 0; q < PER; q++) {
     const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
     if (e >= e1) continue;
     const double *xr = d.x + (int64_t)c[q] * d.ldx;
     const double *dr = d.dy + (int64_t)srow[e - e0] * d.lddy;
-    double s = dr[0] * xr[0];
-    for (int32_t j = 1; j < d.k; j++) s = __builtin_fma(dr[j], xr[j], s);
+    double s = -0.0;
+    int32_t j = 0;
+    for (; j + KC <= d.k; j += KC) {
+      double xv[KC], dyv[KC];
+#pragma unroll
+      for (int u = 0; u < KC; u++) {
+        xv[u] = xr[j + u];
+        dyv[u] = dr[j + u];
+      }
+#pragma unroll
+      for (int u = 0; u < KC; u++) s = __builtin_fma(dyv[u], xv[u], s);
+    }
+    if (j < d.k) {
+      double xv[KC - 1], dyv[KC - 1];
+#pragma unroll
+      for (int u = 0; u < KC - 1; u++)
+        if (j + u < d.k) {
+          xv[u] = xr[j + u];
+          dyv[u] = dr[j + u];
+        }
+#pragma unroll
+      for (int u = 0; u < KC - 1; u++)
+        if (j + u < d.k) s = __builtin_fma(dyv[u], xv[u], s);
+    }
     d.dv[e] = s;
   }
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(kDvThreads) void dvals_kernel(DvArgs d) {
   __shared__ int32_t srow[kDvTile];
-  dvals_tile(d, (int32_t)blockIdx.x, srow);
+  dvals_tile<MULTI>(d, (int32_t)blockIdx.x, srow);
 }
 
 // dx = A^T dy and dvals in one launch, for layouts of short groups: the
@@ -499,7 +529,7 @@ __global__ __launch_bounds__(kDvThreads) void backward_kernel(SpmvArgs a, int32_
   const int64_t before = b * ndv / T;
   const int64_t upto = (b + 1) * ndv / T;
   if (upto > before)
-    dvals_tile(d, (int32_t)before, srow);
+    dvals_tile<false>(d, (int32_t)before, srow);  // k = 1
   else
     direct_block<SPLIT>(a, (int32_t)(b - before), nwork);
 }
@@ -897,7 +927,10 @@ void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *t
   const int64_t blocks = (nnz + kDvTile - 1) / kDvTile;
   if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
   const DvArgs d{rowptr, colind, tile_row, nnz, dy, lddy, x, ldx, k, dvals};
-  hipLaunchKernelGGL(dvals_kernel, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
+  if (k == 1)
+    hipLaunchKernelGGL(dvals_kernel<false>, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
+  else
+    hipLaunchKernelGGL(dvals_kernel<true>, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
   hip_check(hipGetLastError(), "dvals launch");
 }
 
