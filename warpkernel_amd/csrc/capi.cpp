@@ -328,6 +328,11 @@ int wk_engine_create(const wk_layout *L, const int32_t *rowptr_host, const int32
         }
         tile_row[nt] = L->nrows > 0 ? L->nrows - 1 : 0;
         E->d_tile_row = dev_upload(E, tile_row.data(), nt + 1, s);
+        // the row of every CSR entry (dvals)
+        std::vector<int32_t> coo_row((size_t)L->nz);
+        for (int32_t rr = 0; rr < L->nrows; rr++)
+          for (int64_t e = rowptr_host[rr]; e < rowptr_host[rr + 1]; e++) coo_row[e] = rr;
+        E->d_rowind = dev_upload(E, coo_row.data(), L->nz, s);
         hip_check(hipStreamSynchronize(s), "upload");
       }
     } catch (...) {
@@ -461,7 +466,7 @@ int wk_dvals(wk_engine *E, const double *dy_dev, int64_t lddy, const double *x_d
     if (!E->has_adj) fail(WK_ERR_STATE, "engine was created without WK_ENGINE_ADJOINT");
     if (k < 1 || lddy < k || ldx < k) fail(WK_ERR_ARG, "bad k or leading dimension");
     if (E->nnz && (!dy_dev || !x_dev || !dvals_dev)) fail(WK_ERR_ARG, "NULL vector");
-    launch_dvals(E->d_rowptr, E->d_colind, E->d_tile_row, E->nrows, E->nnz, dy_dev, lddy, x_dev,
+    launch_dvals(E->d_rowptr, E->d_colind, E->d_tile_row, E->d_rowind, E->nnz, dy_dev, lddy, x_dev,
                  ldx, k, dvals_dev, static_cast<hipStream_t>(stream));
   });
 }
@@ -488,7 +493,7 @@ int wk_backward(wk_engine *E, const wk_launch *cfg, const void *dy_dev, int64_t 
     }
     const wk_launch c = resolve(E, WK_OP_ADJOINT, k, cfg);
     launch_spmv(E->adj, E->dtype, c, OUT_RUN, dy_dev, lddy, dx_dev, lddx, nullptr, 0, k, s);
-    launch_dvals(E->d_rowptr, E->d_colind, E->d_tile_row, E->nrows, E->nnz,
+    launch_dvals(E->d_rowptr, E->d_colind, E->d_tile_row, E->d_rowind, E->nnz,
                  static_cast<const double *>(dy_dev), lddy, static_cast<const double *>(x_dev), ldx,
                  k, dvals_dev, s);
   });

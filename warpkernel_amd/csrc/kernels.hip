@@ -371,14 +371,17 @@ __global__ void scatter_kernel(const T *__restrict__ src, const uint32_t *__rest
     dst[slot[i]] = src[i];
 }
 
-// dvals[t] = sum_j dy[row_t][j] * x[col_t][j], CSR order.  Each block takes a
-// tile of kDvTile consecutive entries; tile_row[b] (built with the engine) is
-// the row holding the tile's first entry.  The block's threads walk the rows
-// the tile spans and stamp each entry slot of an LDS array with its row, so
-// every entry then finds its row with one conflict-free LDS read.  Column
-// reads and dvals writes are coalesced.  The loads that do not need the rows
-// (columns, and x for k = 1) are issued before the stamping, so their latency
-// overlaps the tile_row -> rowptr chain.
+// dvals[t] = sum_j dy[row_t][j] * x[col_t][j], CSR order.  With one
+// right-hand side each block takes a tile of kDvTile consecutive entries;
+// tile_row[b] (built with the engine) is the row holding the tile's first
+// entry.  The block's threads walk the rows the tile spans and stamp each
+// entry slot of an LDS array with its row, so every entry then finds its row
+// with one conflict-free LDS read.  Column reads and dvals writes are
+// coalesced.  The loads that do not need the rows (columns and x) are issued
+// before the stamping, so their latency overlaps the tile_row -> rowptr
+// chain.  Reading a stored row per entry instead measured the same at k = 1
+// (it adds 4 bytes per entry); with several right-hand sides it is faster,
+// and dvals_coo_kernel does that.
 constexpr int kDvTile = 2048;
 constexpr int kDvThreads = 256;
 
@@ -393,9 +396,7 @@ struct DvArgs {
   double *dv;
 };
 
-// MULTI = false: one right-hand side only (the caller guarantees d.k == 1),
-// which keeps the k-chunk registers out of the fused backward kernel.
-template <bool MULTI>
+// One right-hand side (several go through dvals_coo_kernel).
 __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_t *srow) {
   constexpr int PER = kDvTile / kDvThreads;
   const int64_t e0 = (int64_t)tile * kDvTile;
@@ -414,7 +415,7 @@ __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_
     const int64_t b = d.rowptr[r + 1] < e1 ? d.rowptr[r + 1] : e1;
     for (int64_t e = a; e < b; e++) srow[e - e0] = r;
   }
-  if (!MULTI || d.k == 1) {
+  {
     double xv[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) xv[q] = d.x[(int64_t)c[q] * d.ldx];
@@ -430,19 +431,32 @@ __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_
       const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
       if (e < e1) d.dv[e] = dvv[q] * xv[q];
     }
-    return;
   }
-  __syncthreads();
-  // k right-hand sides: the first product, then one fma per further pair (by
-  // fma onto -0, as in the SpMV), 8 pairs at a time with all 16 loads of a
-  // chunk issued before its fmas
+}
+
+// dvals with k right-hand sides, from the row of every entry (COO rows): no
+// row lookup, no LDS; for each entry the first product, then one fma per
+// further pair (by fma onto -0, as in the SpMV), 8 pairs per chunk with all
+// 16 loads of a chunk issued before its fmas (hex diffusion 48^3, k=8: 45 us
+// with a per-pair loop, 28.6 through the tiles, 25.2 here).
+constexpr int kCooPer = 8;
+__global__ __launch_bounds__(kDvThreads) void dvals_coo_kernel(DvArgs d, const int32_t *rowind) {
   constexpr int KC = 8;
+  const int64_t e0 = (int64_t)blockIdx.x * (kDvThreads * kCooPer) + threadIdx.x;
+  int32_t r[kCooPer], c[kCooPer];
+#pragma unroll
+  for (int q = 0; q < kCooPer; q++) {
+    const int64_t e = e0 + (int64_t)q * kDvThreads;
+    const bool ok = e < d.nnz;
+    r[q] = ok ? rowind[e] : 0;
+    c[q] = ok ? d.colind[e] : 0;
+  }
 #pragma unroll 2
-  for (int q = 0; q < PER; q++) {
-    const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
-    if (e >= e1) continue;
+  for (int q = 0; q < kCooPer; q++) {
+    const int64_t e = e0 + (int64_t)q * kDvThreads;
+    if (e >= d.nnz) continue;
     const double *xr = d.x + (int64_t)c[q] * d.ldx;
-    const double *dr = d.dy + (int64_t)srow[e - e0] * d.lddy;
+    const double *dr = d.dy + (int64_t)r[q] * d.lddy;
     double s = -0.0;
     int32_t j = 0;
     for (; j + KC <= d.k; j += KC) {
@@ -471,10 +485,9 @@ __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_
   }
 }
 
-template <bool MULTI>
 __global__ __launch_bounds__(kDvThreads) void dvals_kernel(DvArgs d) {
   __shared__ int32_t srow[kDvTile];
-  dvals_tile<MULTI>(d, (int32_t)blockIdx.x, srow);
+  dvals_tile(d, (int32_t)blockIdx.x, srow);
 }
 
 // dx = A^T dy and dvals in one launch, for layouts of short groups: the
@@ -529,7 +542,7 @@ __global__ __launch_bounds__(kDvThreads) void backward_kernel(SpmvArgs a, int32_
   const int64_t before = b * ndv / T;
   const int64_t upto = (b + 1) * ndv / T;
   if (upto > before)
-    dvals_tile<false>(d, (int32_t)before, srow);  // k = 1
+    dvals_tile(d, (int32_t)before, srow);
   else
     direct_block<SPLIT>(a, (int32_t)(b - before), nwork);
 }
@@ -920,17 +933,22 @@ void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const
 }
 
 void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *tile_row,
-                  int32_t nrows, int64_t nnz, const double *dy, int64_t lddy, const double *x,
-                  int64_t ldx, int32_t k, double *dvals, hipStream_t stream) {
-  (void)nrows;
+                  const int32_t *rowind, int64_t nnz, const double *dy, int64_t lddy,
+                  const double *x, int64_t ldx, int32_t k, double *dvals, hipStream_t stream) {
   if (nnz <= 0) return;
-  const int64_t blocks = (nnz + kDvTile - 1) / kDvTile;
-  if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
-  const DvArgs d{rowptr, colind, tile_row, nnz, dy, lddy, x, ldx, k, dvals};
-  if (k == 1)
-    hipLaunchKernelGGL(dvals_kernel<false>, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
-  else
-    hipLaunchKernelGGL(dvals_kernel<true>, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
+  if (k == 1) {
+    const int64_t blocks = (nnz + kDvTile - 1) / kDvTile;
+    if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+    const DvArgs d{rowptr, colind, tile_row, nnz, dy, lddy, x, ldx, k, dvals};
+    hipLaunchKernelGGL(dvals_kernel, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d);
+  } else {
+    const int64_t per = (int64_t)kDvThreads * kCooPer;
+    const int64_t blocks = (nnz + per - 1) / per;
+    if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+    const DvArgs d{rowptr, colind, tile_row, nnz, dy, lddy, x, ldx, k, dvals};
+    hipLaunchKernelGGL(dvals_coo_kernel, dim3((uint32_t)blocks), dim3(kDvThreads), 0, stream, d,
+                       rowind);
+  }
   hip_check(hipGetLastError(), "dvals launch");
 }
 

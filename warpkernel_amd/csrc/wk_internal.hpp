@@ -121,6 +121,7 @@ struct wk_engine {
   int32_t *d_rowptr = nullptr;      // CSR pattern for dvals (adjoint engines)
   int32_t *d_colind = nullptr;
   int32_t *d_tile_row = nullptr;    // dvals tile -> first row
+  int32_t *d_rowind = nullptr;      // row of every CSR entry (dvals)
   std::vector<void *> owned;        // every hipMalloc of this engine
   int device = 0;
   std::mutex cfg_mu;                // guards launch_cfg
@@ -165,11 +166,12 @@ void check_index_range_device(const int32_t *a, int64_t n, int32_t bound, hipStr
 // and xcd_remap.
 void launch_rowsum(const DevLayout &D, int32_t dtype, const wk_launch &cfg, double *y,
                    hipStream_t stream);
+// dvals in CSR order: k = 1 on tiles of consecutive entries (tile_row[t] =
+// row of entry t*tile_size, tile_row[tiles] = nrows - 1), k > 1 from the row
+// of every entry (rowind[e]).
 void launch_dvals(const int32_t *rowptr, const int32_t *colind, const int32_t *tile_row,
-                  int32_t nrows, int64_t nnz, const double *dy, int64_t lddy, const double *x,
-                  int64_t ldx, int32_t k, double *dvals, hipStream_t stream);
-// dvals works on tiles of consecutive entries; tile_row[t] = row of entry
-// t*tile_size, tile_row[tiles] = nrows - 1.
+                  const int32_t *rowind, int64_t nnz, const double *dy, int64_t lddy,
+                  const double *x, int64_t ldx, int32_t k, double *dvals, hipStream_t stream);
 // dx = A^T dy and dvals in one launch (FP64, one contiguous right-hand side,
 // group width 64), for layouts whose groups are short.
 void launch_backward(const DevLayout &T, const double *dy, double *dx, const int32_t *rowptr,
