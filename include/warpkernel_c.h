@@ -258,7 +258,11 @@ typedef struct wk_launch {
                               k_tiles 2, 4 and 8 (unroll 8), all k-tiles in
                               one launch; 4: as 3, with values and window
                               positions loaded straight into registers
-                              instead of through LDS; otherwise ignored */
+                              instead of through LDS; 5 (k_tile 1): as 3,
+                              with window positions packed 10 bits each
+                              where every window has at most 1024 entries
+                              (unroll 8: 12-step batches, 16: 24-step),
+                              otherwise as 3; otherwise ignored */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);

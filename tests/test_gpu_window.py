@@ -65,14 +65,16 @@ def check_windows(eng, L, cols):
 
 @pytest.mark.parametrize("name,th", [("hex_elastic_3d_32", 27), ("tet_poisson_3d_96", 5),
                                       ("tet_poisson_3d_96_perm", 5), ("q1_laplace_2d_128", 4)])
-@pytest.mark.parametrize("u,wpb", [(8, 4), (16, 1), (8, 2)])
-def test_fe_window_run_run_x_residual(name, th, u, wpb):
-    say(f"[gpu] window {name} u={u} wpb={wpb}")
+@pytest.mark.parametrize("u,wpb,st", [(8, 4, 3), (16, 1, 3), (8, 2, 3), (16, 1, 5), (8, 4, 5)])
+def test_fe_window_run_run_x_residual(name, th, u, wpb, st):
+    """stage 5: positions packed 10 bits each where every window fits 1024
+    entries (all four matrices), else the stage 3 kernel."""
+    say(f"[gpu] window {name} u={u} wpb={wpb} stage={st}")
     m = fe.CONFIGS[name]()
     rng = np.random.default_rng(21)
     x = rng.uniform(-1, 1, m.ncols)
     bvec = rng.uniform(-1, 1, m.nrows)
-    cfg = dict(WIN, unroll=u, waves_per_block=wpb)
+    cfg = dict(WIN, unroll=u, waves_per_block=wpb, stage_lds=st)
     for variant in (1, 2):
         if variant == 1:
             s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
@@ -110,9 +112,10 @@ def test_random_window_or_fallback(mode):
     ref = O.csr_spmv(rp, cols, vals, x)
     s = structure(64).scan(int(rp[-1]), n, vals, rp, cols)
     eng = engine(s, vals, cols, kr=True)
-    y = nan_vec(n)
-    eng.launch(dev(x), y, cfg=dict(WIN, unroll=8, waves_per_block=4))
-    assert np.array_equal(host(y), ref)
+    for st in (3, 5):  # packed positions only where every window fits 1024 entries
+        y = nan_vec(n)
+        eng.launch(dev(x), y, cfg=dict(WIN, unroll=8, waves_per_block=4, stage_lds=st))
+        assert np.array_equal(host(y), ref), st
     if eng.download(capi.WK_ENGINE_WIN_OFF).size:
         check_windows(eng, O.scan1(rp, 64), cols)
     eng.close()
@@ -128,6 +131,9 @@ def test_fp32_window_matches_staged():
         y2, y3 = nan_vec(m.nrows), nan_vec(m.nrows)
         eng.launch(x, y2, cfg=dict(WIN, stage_lds=2, unroll=8, waves_per_block=4, accumulate=acc))
         eng.launch(x, y3, cfg=dict(WIN, unroll=8, waves_per_block=4, accumulate=acc))
+        assert torch.equal(y2, y3)
+        y3.fill_(float("nan"))
+        eng.launch(x, y3, cfg=dict(WIN, stage_lds=5, unroll=16, waves_per_block=1, accumulate=acc))
         assert torch.equal(y2, y3)
     eng.close()
 

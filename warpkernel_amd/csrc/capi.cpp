@@ -100,6 +100,7 @@ void upload_layout(wk_engine *E, DevLayout &D, const wk_layout &L, const double 
         D.win_cols_kr = dev_upload(E, wc.data(), (int64_t)wc.size(), s);
         hip_check(hipStreamSynchronize(s), "upload");
       }
+      build_packed_positions(E, D, s);
     } else {
       D.win_max = 0;
     }

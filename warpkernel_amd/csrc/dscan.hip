@@ -498,6 +498,7 @@ void device_scan(wk_engine *E, int32_t variant, int32_t threshold, int32_t max_t
       hip_check(hipStreamSynchronize(s), "window write");
       D.win_max = most;
       D.win_total = off[G];
+      build_packed_positions(E, D, s);
     }
   }
 

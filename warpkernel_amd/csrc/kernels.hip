@@ -240,6 +240,8 @@ struct WinArgs {
   const int32_t *cols;   // window columns in the product's numbering
   int32_t wave_bytes;    // LDS per wave: window, then slice
   int32_t win_bytes;     // window part (16-byte multiple)
+  const uint32_t *pk;      // packed positions (spmv_winpk_kernel)
+  const uint32_t *pk_off;  // [ngroups+1] first 12-step block of each group
 };
 
 // Work block -> (group block, k-tile), k-tile fastest: the blocks of one
@@ -333,6 +335,61 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
     else
       a.y[dst * a.ldy + c0 + t] = res;
   }
+}
+
+// One right-hand side through the x window with packed 10-bit positions:
+// spmv_win_kernel<P, SPLIT, OUT, U, 1, 1, 1> with batches of NB 12-step
+// blocks whose positions (16 bytes per lane per block) go straight into
+// registers; the index stream shrinks from 2 to 1.33 bytes per slot.
+template <class P, bool SPLIT, int OUT, int NB>
+__global__ __launch_bounds__(kMaxThreads) void spmv_winpk_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
+                                                                 int32_t remap) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  constexpr int W = 64;
+  constexpr int U = WinBatchPk<P, NB>::U;
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t wpb = blockDim.x >> 6;
+  const int32_t bid = blockIdx.x;
+  if (bid >= nwork) {
+    const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
+    if (pos >= a.nrows) return;
+    const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
+    a.y[dst * a.ldy] = OUT == OUT_RESID ? a.b[dst * a.ldb] : 0.0;
+    return;
+  }
+  const int32_t wib = threadIdx.x >> 6;
+  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(bid, nwork, remap) * wpb + wib);
+  if (wave >= a.ngroups) return;
+  const GroupMeta m = load_meta<W, SPLIT>(a, wave);
+  const V *vbase = static_cast<const V *>(a.vals) + m.base;
+  const u32x4 *pk = reinterpret_cast<const u32x4 *>(w.pk) + (int64_t)w.pk_off[wave] * 64;
+  char *xs = lds + wib * w.wave_bytes;
+  const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
+  WinFill<X, 1> f;
+  f.cols(w.cols + w0, w1 - w0, lane);
+  WinBatchPk<P, NB> bt;
+  const bool early = m.len >= (uint32_t)U;
+  if (early) bt.issue(vbase, pk, 0, lane);  // in flight while the window fills
+  f.gather(xs, w.cols + w0, w1 - w0, static_cast<const X *>(a.x), a.ldx, lane);
+  Acc acc;
+  window_group_pk<P, NB>(acc, vbase, pk, xs, m.len, bt, early, xs + w.win_bytes, lane);
+  if constexpr (SPLIT) {
+    for (uint32_t o = 1; o < m.tpr; o <<= 1) {
+      Acc other = P::shfl_down(acc, (int)o, W);
+      P::add(acc, other);
+    }
+  }
+  const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
+  if (((uint32_t)lane & (m.tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
+  const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
+  const double res = P::result(acc);
+  if constexpr (OUT == OUT_RESID)
+    a.y[dst * a.ldy] = a.b[dst * a.ldb] - res;
+  else
+    a.y[dst * a.ldy] = res;
 }
 
 // ------------------------------------------------------------ small kernels
@@ -746,6 +803,37 @@ WinFn pick_win(int prec, bool split, int out, int u, int kt, int wpw, bool regs)
   return pick_win_split<PF32D>(split, out, u, kt, wpw, regs);
 }
 
+template <class P, bool SPLIT, int OUT, int NB>
+void launch_winpk(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
+                  int32_t nwork, int32_t remap, hipStream_t s) {
+  hipLaunchKernelGGL((spmv_winpk_kernel<P, SPLIT, OUT, NB>), grid, block, lds, s, a, w, nwork,
+                     remap);
+}
+
+using WinPkFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
+                         hipStream_t);
+
+// nb: 12-step blocks per batch (1 or 2)
+template <class P, bool SPLIT, int NB>
+WinPkFn pick_winpk_out(int out) {
+  if (out == OUT_RUN) return &launch_winpk<P, SPLIT, OUT_RUN, NB>;
+  if (out == OUT_RUNX) return &launch_winpk<P, SPLIT, OUT_RUNX, NB>;
+  if constexpr (std::is_same_v<P, PF64>) return &launch_winpk<P, SPLIT, OUT_RESID, NB>;
+  return nullptr;
+}
+
+template <class P, int NB>
+WinPkFn pick_winpk_split(bool split, int out) {
+  return split ? pick_winpk_out<P, true, NB>(out) : pick_winpk_out<P, false, NB>(out);
+}
+
+WinPkFn pick_winpk(int prec, bool split, int out, int nb) {
+  if (nb != 1 && nb != 2) return nullptr;
+  if (prec == 0) return nb == 1 ? pick_winpk_split<PF64, 1>(split, out) : pick_winpk_split<PF64, 2>(split, out);
+  if (prec == 1) return nb == 1 ? pick_winpk_split<PF32C, 1>(split, out) : pick_winpk_split<PF32C, 2>(split, out);
+  return nb == 1 ? pick_winpk_split<PF32D, 1>(split, out) : pick_winpk_split<PF32D, 2>(split, out);
+}
+
 size_t staged_lds_per_wave(int prec, int u) {
   return (size_t)u * 64 * ((prec == 0 ? sizeof(double) : sizeof(float)) + sizeof(int32_t));
 }
@@ -762,7 +850,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
                  int64_t ldx, double *y, int64_t ldy, const double *b, int64_t ldb, int32_t k,
                  hipStream_t stream) {
   if (k < 1) fail(WK_ERR_ARG, "k must be >= 1");
-  if (cfg.stage_lds < 0 || cfg.stage_lds > 4) fail(WK_ERR_ARG, "stage_lds must be 0, 1, 2, 3 or 4");
+  if (cfg.stage_lds < 0 || cfg.stage_lds > 5) fail(WK_ERR_ARG, "stage_lds must be 0 to 5");
   const int wpb = cfg.waves_per_block;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) fail(WK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
   if (cfg.k_tile < 1 || cfg.k_tile > 32 || (cfg.k_tile & (cfg.k_tile - 1)))
@@ -813,6 +901,40 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const bool windowed = cfg.stage_lds >= 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
                           cfg.nontemporal &&
                           (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
+    if (windowed && cfg.stage_lds == 5 && kt == 1 && D.win_pk) {
+      // packed positions: per wave the window, then a 12-step slice of values
+      const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
+      const size_t win_bytes = ((size_t)D.win_max * xsz + 15) / 16 * 16;
+      // unroll 8: 12-step batches (one packed block), unroll 16: 24-step batches
+      const int nb = cfg.unroll == 16 ? 2 : 1;
+      const size_t wave_bytes = win_bytes + (size_t)12 * nb * 64 * vbytes;
+      const size_t need = wave_bytes * (size_t)wpb;
+      WinPkFn wf = pick_winpk(prec, split, out, nb);
+      if (wf && need <= (size_t)lds_limits().per_block) {
+        WinArgs w{};
+        w.off = D.win_off;
+        w.cols = (out == OUT_RUNX && D.win_cols_kr) ? D.win_cols_kr : D.win_cols;
+        w.wave_bytes = (int32_t)wave_bytes;
+        w.win_bytes = (int32_t)win_bytes;
+        w.pk = D.win_pk;
+        w.pk_off = D.win_pk_off;
+        const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
+        const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+        if (nb + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+        for (int32_t t = 0; t < ntiles; t++) {
+          a.x = static_cast<const char *>(x) + (size_t)(j0 + t) * xsz;
+          a.y = y + j0 + t;
+          a.b = b ? b + j0 + t : nullptr;
+          a.k = 1;
+          if (nb + nf > 0)
+            wf(a, w, dim3((uint32_t)(nb + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nb,
+               cfg.xcd_remap ? 1 : 0, stream);
+          hip_check(hipGetLastError(), "spmv launch");
+        }
+        j0 += ntiles;
+        continue;
+      }
+    }
     if (windowed) {
       // k-tile 1: per wave, the window (win_max x kt entries, 16-byte
       // multiple), then a U-step slice of values and 16-bit positions, one
@@ -904,6 +1026,49 @@ void launch_stream_read(const void *p, int64_t bytes, uint32_t *sink, hipStream_
   hipLaunchKernelGGL(stream_read_kernel, dim3(16384), dim3(256), 0, stream,
                      static_cast<const u32x4 *>(p), bytes / 16, sink);
   hip_check(hipGetLastError(), "stream read launch");
+}
+
+// One wave per group: lane l packs its positions of steps 12b .. 12b+11 into
+// four words, three 10-bit fields each (steps past the group's length are 0).
+__global__ __launch_bounds__(256) void pack_positions_kernel(const uint16_t *__restrict__ idx,
+                                                             const int64_t *__restrict__ warp_offsets,
+                                                             const uint32_t *__restrict__ max_nz_rows,
+                                                             const uint32_t *__restrict__ pk_off,
+                                                             int32_t ngroups, u32x4 *__restrict__ out) {
+  const int32_t g = (int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int32_t lane = threadIdx.x & 63;
+  if (g >= ngroups) return;
+  const int64_t base = warp_offsets[g];
+  const uint32_t len = max_nz_rows[g];
+  const uint32_t nbat = pk_off[g + 1] - pk_off[g];
+  for (uint32_t b = 0; b < nbat; b++) {
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < 12; u++) {
+      const uint32_t i = b * 12 + u;
+      if (i < len) w[u / 3] |= (uint32_t)idx[base + lane + 64 * (int64_t)i] << (10 * (u % 3));
+    }
+    out[((int64_t)pk_off[g] + b) * 64 + lane] = u32x4{w[0], w[1], w[2], w[3]};
+  }
+}
+
+void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream) {
+  D.win_pk = nullptr;
+  D.win_pk_off = nullptr;
+  if (D.W != 64 || D.win_max <= 0 || D.win_max > kPackedWindow || !D.win_idx || D.ngroups <= 0)
+    return;
+  std::vector<uint32_t> off((size_t)D.ngroups + 1, 0);
+  for (int32_t g = 0; g < D.ngroups; g++) off[g + 1] = off[g] + (D.h_max_nz_rows[g] + 11) / 12;
+  D.win_pk_off = dev_upload(E, off.data(), (int64_t)off.size(), stream);
+  D.win_pk = dev_alloc<uint32_t>(E, (int64_t)off.back() * 64 * 4);
+  constexpr int wpb = 4;
+  const int64_t blocks = ((int64_t)D.ngroups + wpb - 1) / wpb;
+  if (D.win_pk)
+    hipLaunchKernelGGL(pack_positions_kernel, dim3((uint32_t)blocks), dim3(wpb * 64), 0, stream,
+                       D.win_idx, D.warp_offsets, D.max_nz_rows, D.win_pk_off, D.ngroups,
+                       reinterpret_cast<u32x4 *>(D.win_pk));
+  hip_check(hipGetLastError(), "pack launch");
+  hip_check(hipStreamSynchronize(stream), "pack");
 }
 
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream) {

@@ -510,6 +510,105 @@ __device__ __forceinline__ void window_group(typename P::Acc (&acc)[KT], const t
   }
 }
 
+// Window positions packed 10 bits each (three per 32-bit word): a 12-step
+// block of one lane is one 16-byte word group, loaded straight into registers
+// (1.33 bytes per slot instead of 2), while the batch's values are staged
+// through the slice as in WinBatch.  A batch is NB blocks (U = 12*NB steps).
+// The position loads go out first, so the wait for the values covers them.
+template <class P, int NB>
+struct WinBatchPk {
+  static constexpr int U = 12 * NB;
+  static constexpr int NV = U * 64 * (int)sizeof(typename P::V) / 1024;
+  static_assert(NV * 1024 == U * 64 * (int)sizeof(typename P::V), "whole 1 KB chunks");
+  u32x4 tv[NV], tp[NB];
+  __device__ __forceinline__ void issue(const typename P::V *vbase, const u32x4 *pk, uint32_t b,
+                                        int32_t lane) {
+#pragma unroll
+    for (int q = 0; q < NB; q++) tp[q] = pk[((int64_t)b * NB + q) * 64 + lane];
+    const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)b * U * 64);
+#pragma unroll
+    for (int q = 0; q < NV; q++) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+  }
+  __device__ __forceinline__ void stage(char *slice, int32_t lane) const {
+    u32x4 *lv = reinterpret_cast<u32x4 *>(slice);
+#pragma unroll
+    for (int q = 0; q < NV; q++) lv[q * 64 + lane] = tv[q];
+  }
+  // position of step u of a block (u is a constant after unrolling)
+  __device__ static __forceinline__ uint32_t pos(const u32x4 &w, int u) {
+    return (w[u / 3] >> (10 * (u % 3))) & 1023u;
+  }
+};
+
+// The U steps of one staged batch with packed positions: value from the
+// slice, x from the window.
+template <class P, int NB>
+__device__ __forceinline__ void pk_batch(typename P::Acc &acc, const typename P::V *lv,
+                                         const char *xs, const u32x4 (&w)[NB], int32_t lane) {
+  constexpr int U = 12 * NB;
+  typename P::X xv[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    typename P::X e[1];
+    WinEntry<typename P::X, 1>::read(xs, WinBatchPk<P, NB>::pos(w[u / 12], u % 12), e);
+    xv[u] = e[0];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+}
+
+// One group through its x window with packed positions (one right-hand side):
+// the same products in the same order as window_group.  When `issued`, the
+// caller has issued batch 0 into bt before filling the window.
+template <class P, int NB>
+__device__ __forceinline__ void window_group_pk(typename P::Acc &acc, const typename P::V *vbase,
+                                                const u32x4 *pk, const char *xs, uint32_t len,
+                                                WinBatchPk<P, NB> &bt, bool issued, char *slice,
+                                                int32_t lane) {
+  using V = typename P::V;
+  constexpr int U = WinBatchPk<P, NB>::U;
+  const V *lv = reinterpret_cast<const V *>(slice);
+  P::init(acc);  // every slot by fma onto -0
+  uint32_t i = 0, b = 0;
+  if (i + U <= len) {
+    if (!issued) bt.issue(vbase, pk, 0, lane);
+    for (; i + 2 * U <= len; i += U, b++) {
+      bt.stage(slice, lane);
+      u32x4 w[NB];
+#pragma unroll
+      for (int q = 0; q < NB; q++) w[q] = bt.tp[q];
+      bt.issue(vbase, pk, b + 1, lane);
+      pk_batch<P, NB>(acc, lv, xs, w, lane);
+    }
+    bt.stage(slice, lane);
+    u32x4 w[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) w[q] = bt.tp[q];
+    pk_batch<P, NB>(acc, lv, xs, w, lane);
+    i += U;
+    b++;
+  }
+  // the last, partial batch: its packed blocks (those the group has) and
+  // direct value loads
+  if (i < len) {
+    u32x4 w[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+      if (i + 12 * q < len) w[q] = pk[((int64_t)b * NB + q) * 64 + lane];
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) v[u] = __builtin_nontemporal_load(vbase + (int64_t)(i + u) * 64 + lane);
+#pragma unroll
+    for (int u = 0; u < U - 1; u++)
+      if (i + u < len) {
+        typename P::X xv[1];
+        WinEntry<typename P::X, 1>::read(xs, WinBatchPk<P, NB>::pos(w[u / 12], u % 12), xv);
+        P::step(acc, v[u], xv[0]);
+      }
+  }
+}
+
 // The x reads and FMAs of one staged batch: slot u of the lane reads its
 // value and window position from the slice, then its KW x values (values
 // t0 .. t0+KW-1 of the entry).

@@ -47,10 +47,15 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
               c.stage_lds = st;
               out.push_back(c);
             }
-            // x windows (the launcher falls back to 2 where a window does not fit)
+            // x windows, with 16-bit or packed 10-bit positions (the launcher
+            // falls back where a window does not fit)
             if (D.win_max > 0) {
               c.stage_lds = 3;
               out.push_back(c);
+              if (D.win_pk) {
+                c.stage_lds = 5;
+                out.push_back(c);
+              }
             }
             c.stage_lds = 0;
           }
@@ -107,8 +112,11 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     // block, a contiguous range of groups per XCD, and 16-step batches for
     // long groups (hex elasticity 64^3: 99.3 us against 104 with 8-step
     // batches and no remap; WPK2 at threshold 27, 21-step groups: 8-step
-    // batches, 109.7 us against 139 with 16)
-    c.stage_lds = 3;
+    // batches, 109.7 us against 139 with 16).  Where every window has at
+    // most 1024 entries the positions are packed 10 bits each, 9.3 bytes per
+    // slot (stage 5: 24- or 12-step batches; 64^3: 96.7 us, 32^3: 13.8
+    // against 15.0, WPK2 64^3: 107.5 against 110.2)
+    c.stage_lds = D.win_pk ? 5 : 3;
     c.unroll = mean_len >= 40.0 ? 16 : 8;
     c.waves_per_block = 1;
     c.xcd_remap = 1;

@@ -93,7 +93,15 @@ struct DevLayout {
   int32_t *win_cols_kr = nullptr;  // the same columns in Kr numbering (row_map_inv)
   int32_t win_max = 0;             // largest window; 0 when the layout has none
   int64_t win_total = 0;           // win_off[ngroups]: entries over all windows
+  // the same positions packed 10 bits each, three per 32-bit word, when every
+  // window has at most 1024 entries: per group ceil(len/12) blocks of 64 lanes
+  // x 16 bytes, lane l's block holding its 12 positions of those 12 steps
+  uint32_t *win_pk = nullptr;      // [win_pk_off[ngroups] * 64 * 4]
+  uint32_t *win_pk_off = nullptr;  // [ngroups+1] first 12-step block of each group
 };
+
+// Window positions fit the packed form up to this many entries per window.
+constexpr int32_t kPackedWindow = 1024;
 
 // Largest x window (distinct columns of one group) that the window kernel
 // stages; layouts with a larger group get no windows.
@@ -158,6 +166,9 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
 void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
                            hipStream_t stream);
 void launch_stream_read(const void *p, int64_t bytes, uint32_t *sink, hipStream_t stream);
+// Builds D.win_pk / D.win_pk_off from D.win_idx when the windows fit the
+// packed form (width 64, win_max <= kPackedWindow); otherwise leaves them null.
+void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream);
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
 // Fails with WK_ERR_ARG unless every a[i], i < n, lies in [0, bound); device
 // array, synchronises the stream.
