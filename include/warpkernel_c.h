@@ -237,7 +237,8 @@ typedef struct wk_launch {
                               block's waves share one group's x window, each
                               summing an equal share of the k-tile's
                               right-hand sides (at most one wave per 16
-                              bytes of a window entry) */
+                              bytes of a window entry; FP32 engines with
+                              stage_lds 4: at most one per right-hand side) */
   int32_t unroll;          /* slots in flight per lane: 4, 8 or 16 (k_tile 1) */
   int32_t k_tile;          /* right-hand sides per pass: 1, 2, 4, 8, 16 or 32 */
   int32_t xcd_remap;       /* 1: contiguous group ranges per XCD */

@@ -755,19 +755,33 @@ WinFn pick_win_d(int u, int kt, int wpw) {
     return nullptr;
   }
   if (u != 8) return nullptr;
-  if (kt == 2) return wpw == 1 ? &launch_win<P, SPLIT, OUT, 8, 2, D, 1> : nullptr;
+  // FP32, register-fed: down to one right-hand side per wave
+  constexpr bool fine = xsz == 4 && D == 0;
+  if (kt == 2) {
+    if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 2, D, 1>;
+    if constexpr (fine) {
+      if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 2, D, 2>;
+    }
+    return nullptr;
+  }
   if (kt == 4) {
     if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 4, D, 1>;
-    if constexpr (2 * xsz >= 16) {
+    if constexpr (2 * xsz >= 16 || fine) {
       if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 4, D, 2>;
+    }
+    if constexpr (fine) {
+      if (wpw == 4) return &launch_win<P, SPLIT, OUT, 8, 4, D, 4>;
     }
     return nullptr;
   }
   if (kt == 8) {
     if (wpw == 1) return &launch_win<P, SPLIT, OUT, 8, 8, D, 1>;
     if (wpw == 2) return &launch_win<P, SPLIT, OUT, 8, 8, D, 2>;
-    if constexpr (2 * xsz >= 16) {
+    if constexpr (2 * xsz >= 16 || fine) {
       if (wpw == 4) return &launch_win<P, SPLIT, OUT, 8, 8, D, 4>;
+    }
+    if constexpr (fine) {
+      if (wpw == 8) return &launch_win<P, SPLIT, OUT, 8, 8, D, 8>;
     }
     return nullptr;
   }
@@ -941,13 +955,14 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       // group per wave.  Wider k-tiles: the block's waves share one group's
       // window, each with its own slice.
       const int u = kt == 1 ? cfg.unroll : 8;
-      // waves per window: waves_per_block, at most one per 16-byte chunk
-      const int chunks = std::max(1, kt * (int)xsz / 16);
+      // waves per window: waves_per_block, at most one per 16-byte chunk of
+      // an entry, or (FP32, values in registers) one per right-hand side
+      const bool regs = cfg.stage_lds == 4;  // no slices
+      const int chunks = regs && xsz == 4 ? kt : std::max(1, kt * (int)xsz / 16);
       const int wpw = kt == 1 ? 1 : std::min(wpb, chunks);
       const int bw = wpw > 1 ? wpw : wpb;  // waves per block
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
       const size_t win_bytes = ((size_t)D.win_max * kt * xsz + 15) / 16 * 16;
-      const bool regs = cfg.stage_lds == 4;  // no slices
       const size_t slice_bytes = regs ? 0 : (size_t)u * 64 * (vbytes + sizeof(uint16_t));
       const size_t wave_bytes = win_bytes + slice_bytes;
       const size_t need = wpw > 1 ? win_bytes + slice_bytes * (size_t)wpw : wave_bytes * (size_t)wpb;

@@ -349,13 +349,24 @@ struct WinEntry {
       }
     }
   }
-  // values [t0, t0 + KW) of entry j, where KW values fill whole 16-byte
-  // chunks and t0 is a multiple of KW (KW = KT: the whole entry)
+  // values [t0, t0 + KW) of entry j, t0 a multiple of KW (KW = KT: the whole
+  // entry): whole 16-byte chunks, or a 4- or 8-byte part of one chunk
   template <int KW>
   __device__ static __forceinline__ void read_part(const char *xs, uint32_t j, uint32_t t0,
                                                    X (&v)[KW]) {
     if constexpr (KW == KT) {
       read(xs, j, v);
+    } else if constexpr (KW * sizeof(X) < 16) {
+      constexpr uint32_t pb = KW * (uint32_t)sizeof(X);
+      static_assert(pb == 4 || pb == 8, "4- or 8-byte parts");
+      const uint32_t b0 = t0 * (uint32_t)sizeof(X);
+      const char *p = xs + j * kBytes + 16u * ((b0 >> 4) ^ swz(j)) + (b0 & 15u);
+      if constexpr (pb == 8) {
+        const uint2 c = *reinterpret_cast<const uint2 *>(p);
+        __builtin_memcpy(&v[0], &c, 8);
+      } else {
+        v[0] = *reinterpret_cast<const X *>(p);
+      }
     } else {
       static_assert(KW * sizeof(X) % 16 == 0 && kChunks > 1, "whole chunks of a chunked entry");
       constexpr uint32_t nq = KW * sizeof(X) / 16;

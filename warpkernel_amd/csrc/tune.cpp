@@ -129,8 +129,11 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     // elasticity 32^3 and 64^3: FP64 k=4 17.3 / 119 us (stage 3: 23.7 /
     // 128), k=8 and 16 with k-tile 8 shared by two waves; at k=32 the
     // staged form with one wave per window stays ahead (111 / 817 us).  FP32
-    // is bound by its arithmetic: k-tile 8 shared by two waves from k=8
-    // (k=32: 90 / 626 us), four windows per block below.
+    // (compensated) is bound by its arithmetic; on 32^3 a window per wave up
+    // to k = 8 (k=2 11.2, k=4 15.6, k=8 23.0 us, against 12.7 / 19.0 / 24.9
+    // with windows shared one right-hand side per wave), and from k = 16 four
+    // waves sharing a k-tile-8 window, two right-hand sides each (k=16 46.2
+    // against 49.4 with two waves, k=32 90.5 against 89.0).
     c.unroll = 8;
     c.xcd_remap = 1;
     if (dtype == WK_F64) {
@@ -151,13 +154,8 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
       }
     } else {
       c.stage_lds = 4;
-      if (k < 8) {
-        c.k_tile = pow2_at_most(k);
-        c.waves_per_block = 4;
-      } else {
-        c.k_tile = 8;
-        c.waves_per_block = 2;
-      }
+      c.k_tile = std::min(8, pow2_at_most(k));
+      c.waves_per_block = k < 16 ? 1 : 4;
     }
   } else if (k == 1 && D.W == 64 && mean_len >= 20.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
