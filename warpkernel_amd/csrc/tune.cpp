@@ -115,9 +115,11 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     // batches, 109.7 us against 139 with 16).  Where every window has at
     // most 1024 entries the positions are packed 10 bits each, 9.3 bytes per
     // slot (stage 5: 24- or 12-step batches; 64^3: 96.7 us, 32^3: 13.8
-    // against 15.0, WPK2 64^3: 107.5 against 110.2)
+    // against 15.0, WPK2 64^3: 107.5 against 110.2).  FP32 values take the
+    // 12-step batches at every length (32^3 compensated 9.3 us against 9.6,
+    // FP64 accumulation 8.8 against 12.1; 64^3 58.4 / 58.5, 58.2 / 71.4)
     c.stage_lds = D.win_pk ? 5 : 3;
-    c.unroll = mean_len >= 40.0 ? 16 : 8;
+    c.unroll = mean_len >= 40.0 && dtype == WK_F64 ? 16 : 8;
     c.waves_per_block = 1;
     c.xcd_remap = 1;
   } else if (k >= 2 && windows) {
