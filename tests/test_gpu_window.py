@@ -196,23 +196,28 @@ def test_multi_rhs_all_paths_bitexact(k):
 @pytest.mark.parametrize("k", [2, 4, 8])
 def test_multi_rhs_fp32_windowed_matches_direct(k):
     say(f"[gpu] multi-RHS fp32 k={k}")
-    """Every window shape, WPK1 and WPK2: with values in registers (stage 4)
-    up to one wave per right-hand side shares a window."""
+    """Every window shape, WPK1 and WPK2, run and run_x: with values in
+    registers (stage 4) up to one wave per right-hand side shares a window."""
     m = fe.CONFIGS["hex_elastic_3d_32"]()
-    x = dev(fe.uniform_x(m.ncols, k, seed=4100 + k).reshape(m.ncols, k).astype(np.float32))
+    xh = fe.uniform_x(m.ncols, k, seed=4100 + k).reshape(m.ncols, k).astype(np.float32)
+    x = dev(xh)
     for s in (structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind),
               structure2(64, 32).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, 27)):
-        eng = engine(s, m.vals, m.colind, dtype="f32")
+        eng = engine(s, m.vals, m.colind, dtype="f32", kr=True)
+        xr = np.empty_like(xh)
+        xr[np.asarray(s.row_map_inv)] = xh  # reorder_x, row by row
+        xrd = dev(xr)
         for acc in (0, 1):
-            y0 = torch.full((m.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
-            y1 = torch.full_like(y0, float("nan"))
-            eng.launch(x, y0, cfg=dict(k_tile=k, stage_lds=0, accumulate=acc))
-            for wpb in (1, 2, 4, 8):
-                for st in (3, 4):
-                    y1.fill_(float("nan"))
-                    eng.launch(x, y1, cfg=dict(WIN, k_tile=k, unroll=8, waves_per_block=wpb,
-                                               accumulate=acc, stage_lds=st))
-                    assert torch.equal(y0, y1), (s.variant, acc, wpb, st)
+            for launch, xin in ((eng.launch, x), (eng.launch_x, xrd)):
+                y0 = torch.full((m.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
+                y1 = torch.full_like(y0, float("nan"))
+                launch(xin, y0, cfg=dict(k_tile=k, stage_lds=0, accumulate=acc))
+                for wpb in (1, 2, 4, 8):
+                    for st in (3, 4):
+                        y1.fill_(float("nan"))
+                        launch(xin, y1, cfg=dict(WIN, k_tile=k, unroll=8, waves_per_block=wpb,
+                                                 accumulate=acc, stage_lds=st))
+                        assert torch.equal(y0, y1), (s.variant, acc, wpb, st, launch.__name__)
         eng.close()
 
 
