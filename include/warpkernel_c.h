@@ -356,10 +356,11 @@ typedef struct wk_batch wk_batch;
  * dispatcher balance them; > 0 (at most 1024) runs that many persistent waves
  * per compute unit, each over a contiguous range of equal cost.  stage_lds: as
  * in wk_launch, for batches with one right-hand side (0 direct loads, 1
- * staged, 2 staged with the next batch in flight, 3 as 2 plus x windows);
- * staging applies to the matrices whose mean group length is at least 24 and
- * windows to those with windows whose entries serve at least 4.5 slots each
- * on average; the others use direct loads. */
+ * staged, 2 staged with the next batch in flight, 3 as 2 plus x windows, 5 as
+ * 3 with packed window positions where a matrix has them, one wave per
+ * block); staging applies to the matrices whose mean group length is at
+ * least 24 and windows to those with windows whose entries serve at least 4.5
+ * slots each on average; the others use direct loads. */
 int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
                     int32_t stage_lds, wk_batch **out);
 

@@ -50,8 +50,9 @@ def make_engines(ms, dtype="f64"):
     return engs
 
 
-@pytest.mark.parametrize("k,stage,wpc", [(1, 0, 0), (1, 1, 0), (1, 2, 0), (1, 3, 0), (4, 3, 0),
-                                         (3, 0, 0), (1, 3, 8), (1, 0, 64), (4, 2, 16)])
+@pytest.mark.parametrize("k,stage,wpc", [(1, 0, 0), (1, 1, 0), (1, 2, 0), (1, 3, 0), (1, 5, 0),
+                                         (4, 3, 0), (4, 5, 0), (3, 0, 0), (1, 3, 8), (1, 5, 8),
+                                         (1, 0, 64), (4, 2, 16)])
 def test_batch_of_64_matrices_bitexact(k, stage, wpc):
     ms = mix()
     say(f"[gpu] batch of {len(ms)} matrices, {sum(m.nnz for m in ms)} nnz, k={k}, stage={stage}, "
@@ -75,16 +76,18 @@ def test_batch_fp32_compensated():
     say("[gpu] batch FP32")
     engs = make_engines(ms, "f32")
     xs_h = [fe.uniform_x(m.ncols, seed=7100 + i).astype(np.float32) for i, m in enumerate(ms)]
-    ys = [torch.empty(m.nrows, dtype=torch.float64, device="cuda") for m in ms]
-    b = batch(engs, waves_per_cu=4)
-    b.bind([dev(x) for x in xs_h], ys)
-    b.run()
-    torch.cuda.synchronize()
-    for i, m in enumerate(ms):
-        vq = m.vals.astype(np.float32).astype(np.float64)
-        ref = O.csr_spmv(m.rowptr, m.colind, vq, xs_h[i].astype(np.float64))
-        y = ys[i].cpu().numpy()
-        assert np.linalg.norm(y - ref) <= 1e-10 * np.linalg.norm(ref)
+    for wpc, stage in ((4, 3), (0, 5)):
+        ys = [torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda") for m in ms]
+        b = batch(engs, waves_per_cu=wpc, stage_lds=stage)
+        b.bind([dev(x) for x in xs_h], ys)
+        b.run()
+        torch.cuda.synchronize()
+        for i, m in enumerate(ms):
+            vq = m.vals.astype(np.float32).astype(np.float64)
+            ref = O.csr_spmv(m.rowptr, m.colind, vq, xs_h[i].astype(np.float64))
+            y = ys[i].cpu().numpy()
+            assert np.linalg.norm(y - ref) <= 1e-10 * np.linalg.norm(ref), (stage, i)
+        b.close()
 
 
 def test_graph_replays_match_eager():

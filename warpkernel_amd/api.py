@@ -492,9 +492,10 @@ class batch:
 
     xs / ys: lists of device tensors, [ncols_i] / [nrows_i] or [n, k].
     waves_per_cu = 0: one wave per group (default); > 0: that many persistent
-    waves per compute unit over equal-cost ranges."""
+    waves per compute unit over equal-cost ranges.  stage_lds 5 (default):
+    x windows with packed positions where a matrix has them."""
 
-    def __init__(self, engines, waves_per_cu: int = 0, stage_lds: int = 3):
+    def __init__(self, engines, waves_per_cu: int = 0, stage_lds: int = 5):
         self.engines = list(engines)  # keep them alive
         arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
         h = C.c_void_p()

@@ -37,9 +37,12 @@ struct BatchDesc {
   int32_t nznrows, nrows, ngroups, split;
   int32_t staged;    // stream this matrix's groups through LDS (long rows)
   int32_t windowed;  // and read x from the group's LDS window (stage 3)
+  int32_t packed;    // windows with packed positions (stage 5)
   const uint16_t *win_idx;
   const uint32_t *win_off;
   const int32_t *win_cols;
+  const uint32_t *win_pk;
+  const uint32_t *win_pk_off;
 };
 
 struct BatchItem {
@@ -73,12 +76,39 @@ struct wk_graph {
 namespace wk {
 namespace {
 
+// Packed-position batches per 12-step block: two for FP64 values, one for FP32
+// (the single-matrix defaults for long groups).
+template <class P>
+constexpr int batch_nb() {
+  return sizeof(typename P::V) == 8 ? 2 : 1;
+}
+
+// LDS slice per wave behind the window: the largest of the staged batch, the
+// window batch (values and 16-bit positions) and, at STAGE 5, the packed batch.
+template <class P, int KT, int U, int STAGE>
+constexpr int batch_slice() {
+  if constexpr (STAGE == 0 || KT != 1) return 16;
+  constexpr int st = staged_bytes<P, U>();
+  constexpr int win = U * 64 * (int)(sizeof(typename P::V) + sizeof(uint16_t));
+  constexpr int pk = STAGE == 5 ? 12 * batch_nb<P>() * 64 * (int)sizeof(typename P::V) : 0;
+  constexpr int a = st > win ? st : win;
+  return a > pk ? a : pk;
+}
+
+// Waves per block: one at STAGE 5 (a window and a packed slice per block, as
+// the single-matrix default), else four.
+template <int STAGE>
+constexpr int batch_wpb() {
+  return STAGE == 5 ? 1 : 4;
+}
+
 // One group of one matrix for right-hand sides [j0, j0+KT): the summation
 // order of warpKernel / warpKernel2 (every slot by fma onto -0, i.e. first
 // product then fma; the pairwise lane tree for split rows).  With STAGE > 0
 // and KT = 1 the group streams through the wave's LDS slice (16-byte loads);
-// with STAGE 3 a windowed matrix reads x from the wave's LDS window (the
-// win_bytes in front of the slice).
+// with STAGE 3 or 5 a windowed matrix reads x from the wave's LDS window (the
+// win_bytes in front of the slice), at STAGE 5 with packed positions where the
+// matrix has them.
 template <class P, int KT, int U, int STAGE>
 __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32_t j0, int32_t lane,
                                             char *region, int32_t win_bytes) {
@@ -95,8 +125,24 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   const X *xb = static_cast<const X *>(d.x) + j0;
   Acc acc[KT];
   bool done = false;
-  if constexpr (STAGE == 3 && KT == 1) {
-    if (d.windowed) {
+  if constexpr (STAGE == 5 && KT == 1) {
+    if (d.windowed && d.packed) {
+      constexpr int NB = batch_nb<P>();
+      char *xs = region;
+      const u32x4 *pk = reinterpret_cast<const u32x4 *>(d.win_pk) + (int64_t)d.win_pk_off[g] * 64;
+      const uint32_t w0 = d.win_off[g], w1 = d.win_off[g + 1];
+      WinFill<X, 1> f;
+      f.cols(d.win_cols + w0, w1 - w0, lane);
+      WinBatchPk<P, NB> bt;
+      const bool early = len >= (uint32_t)WinBatchPk<P, NB>::U;
+      if (early) bt.issue(vbase, pk, 0, lane);  // in flight while the window fills
+      f.gather(xs, d.win_cols + w0, w1 - w0, xb, d.ldx, lane);
+      window_group_pk<P, NB>(acc[0], vbase, pk, xs, len, bt, early, slice, lane);
+      done = true;
+    }
+  }
+  if constexpr ((STAGE == 3 || STAGE == 5) && KT == 1) {
+    if (!done && d.windowed) {
       char *xs = region;
       const uint16_t *ibase = d.win_idx + base;
       WinBatch<P, U> bt;
@@ -110,7 +156,7 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   }
   if constexpr (STAGE > 0 && KT == 1) {
     if (!done && d.staged) {
-      acc[0] = staged_group<P, U, STAGE == 2>(vbase, cbase, xb, d.ldx, len, slice, lane);
+      acc[0] = staged_group<P, U, STAGE >= 2>(vbase, cbase, xb, d.ldx, len, slice, lane);
       done = true;
     }
   }
@@ -149,8 +195,8 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict_
                                                     int32_t nwaves, int32_t ktiles,
                                                     int32_t win_bytes) {
   // wstart == nullptr: wave w runs item w alone.  Per wave, LDS holds the
-  // window (win_bytes, stage 3) and then the staging slice.
-  constexpr int kSlice = (STAGE > 0 && KT == 1) ? staged_bytes<P, U>() : 16;
+  // window (win_bytes, stage 3 and 5) and then the staging slice.
+  constexpr int kSlice = batch_slice<P, KT, U, STAGE>();
   extern __shared__ __attribute__((aligned(16))) char lds[];
   const int32_t region = win_bytes + kSlice;
   const int32_t lane = threadIdx.x & 63;
@@ -181,12 +227,13 @@ template <class P, int KT, int STAGE>
 void launch_batch(const wk_batch *B, int32_t ktiles, hipStream_t s) {
   const bool flat = B->nwaves == 0;
   const int64_t waves = flat ? B->nitems : B->nwaves;
-  const int64_t blocks = (waves + 3) / 4;
+  constexpr int wpb = batch_wpb<STAGE>();
+  const int64_t blocks = (waves + wpb - 1) / wpb;
   if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
-  constexpr int kSlice = (STAGE > 0 && KT == 1) ? staged_bytes<P, 8>() : 16;
-  const int32_t win = STAGE == 3 ? B->win_bytes : 0;
-  const size_t lds = 4 * (size_t)(win + kSlice);
-  hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE>), dim3((uint32_t)blocks), dim3(256), lds, s,
+  constexpr int kSlice = batch_slice<P, KT, 8, STAGE>();
+  const int32_t win = STAGE == 3 || STAGE == 5 ? B->win_bytes : 0;
+  const size_t lds = wpb * (size_t)(win + kSlice);
+  hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE>), dim3((uint32_t)blocks), dim3(wpb * 64), lds, s,
                      B->d_desc, B->d_items, flat ? nullptr : B->d_wstart, (int32_t)waves, ktiles,
                      win);
 }
@@ -195,6 +242,7 @@ template <class P>
 BatchFn pick_batch_kt(int kt, int stage) {
   switch (kt) {
     case 1:
+      if (stage == 5) return &launch_batch<P, 1, 5>;
       if (stage == 3) return &launch_batch<P, 1, 3>;
       if (stage == 2) return &launch_batch<P, 1, 2>;
       if (stage == 1) return &launch_batch<P, 1, 1>;
@@ -227,7 +275,8 @@ int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_
     if (!out || !engines || count < 1) fail(WK_ERR_ARG, "bad argument");
     *out = nullptr;
     if (waves_per_cu < 0 || waves_per_cu > 1024) fail(WK_ERR_ARG, "waves_per_cu must be in [0, 1024]");
-    if (stage_lds < 0 || stage_lds > 3) fail(WK_ERR_ARG, "stage_lds must be 0, 1, 2 or 3");
+    if (stage_lds < 0 || stage_lds > 5 || stage_lds == 4)
+      fail(WK_ERR_ARG, "stage_lds must be 0, 1, 2, 3 or 5");
     auto *B = new wk_batch();
     B->stage = stage_lds;
     try {
@@ -292,11 +341,14 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       d.staged = D.ngroups && mean_len >= 24.0;
       // x windows pay off where each entry serves enough slots (the rule of
       // the single-matrix default)
-      d.windowed = B->stage == 3 && kt == 1 && D.win_max > 0 && D.win_total > 0 &&
-                   (double)D.allocate_nz >= 4.5 * (double)D.win_total;
+      d.windowed = (B->stage == 3 || B->stage == 5) && kt == 1 && D.win_max > 0 &&
+                   D.win_total > 0 && (double)D.allocate_nz >= 4.5 * (double)D.win_total;
+      d.packed = B->stage == 5 && D.win_pk != nullptr;
       d.win_idx = D.win_idx;
       d.win_off = D.win_off;
       d.win_cols = D.win_cols;
+      d.win_pk = D.win_pk;
+      d.win_pk_off = D.win_pk_off;
       if (d.windowed) {
         const int32_t xsz = E->dtype == WK_F64 ? 8 : 4;
         win_bytes = std::max<int32_t>(win_bytes, (D.win_max * xsz + 15) / 16 * 16);
@@ -352,8 +404,14 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
     hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(), hipMemcpyHostToDevice), "bind");
     if (!wstart.empty())
       hip_check(hipMemcpy(B->d_wstart, wstart.data(), sizeof(int32_t) * wstart.size(), hipMemcpyHostToDevice), "bind");
-    // the window and the staging slice of 4 waves must fit one block's LDS
-    if (win_bytes && 4 * (size_t)(win_bytes + staged_bytes<PF64, 8>()) > 65536) {
+    // the windows and slices of a block's waves must fit one block's LDS
+    const bool f64 = B->dtype == WK_F64;
+    const size_t block_lds =
+        B->stage == 5 ? (size_t)batch_wpb<5>() * (win_bytes + (f64 ? batch_slice<PF64, 1, 8, 5>()
+                                                                     : batch_slice<PF32C, 1, 8, 5>()))
+                      : 4 * (size_t)(win_bytes + (f64 ? batch_slice<PF64, 1, 8, 3>()
+                                                        : batch_slice<PF32C, 1, 8, 3>()));
+    if (win_bytes && block_lds > 65536) {
       for (BatchDesc &d : B->desc) d.windowed = 0;
       win_bytes = 0;
       hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(),
