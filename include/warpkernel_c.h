@@ -330,8 +330,9 @@ int wk_dvals(wk_engine *engine, const double *dy_dev, int64_t lddy, const double
  * the WPK2 threshold in its drivers, wpk1.cu:138-257, wpk2.cu:156-302).  Times
  * every candidate launch configuration of operation `op` on this engine with
  * the given device vectors (k right-hand sides, reps launches each, HIP
- * events on `stream`), stores the fastest in the engine for (op, k) and
- * returns it with its mean time.  Only performance knobs vary; base (may be
+ * events on `stream`), times the three fastest twice more (each keeps its best
+ * mean), stores the fastest in the engine for (op, k) and returns it with that
+ * mean time.  Only performance knobs vary; base (may be
  * NULL) fixes the numerics (accumulate).  Results are identical for every
  * candidate.  y_dev is overwritten. */
 int wk_tune(wk_engine *engine, int32_t op, const wk_launch *base, const void *x_dev,
