@@ -754,6 +754,15 @@ WinFn pick_win_d(int u, int kt, int wpw) {
     if (u == 16) return &launch_win<P, SPLIT, OUT, 16, 1, D1, 1>;
     return nullptr;
   }
+  if constexpr (D == 0) {
+    // register-fed, 16-step batches (two alternating: 32 steps in flight)
+    if (u == 16) {
+      if (kt == 4 && wpw == 1) return &launch_win<P, SPLIT, OUT, 16, 4, 0, 1>;
+      if (kt == 8 && wpw == 1) return &launch_win<P, SPLIT, OUT, 16, 8, 0, 1>;
+      if (kt == 8 && wpw == 2) return &launch_win<P, SPLIT, OUT, 16, 8, 0, 2>;
+      return nullptr;
+    }
+  }
   if (u != 8) return nullptr;
   // FP32, register-fed: down to one right-hand side per wave
   constexpr bool fine = xsz == 4 && D == 0;
@@ -954,7 +963,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       // multiple), then a U-step slice of values and 16-bit positions, one
       // group per wave.  Wider k-tiles: the block's waves share one group's
       // window, each with its own slice.
-      const int u = kt == 1 ? cfg.unroll : 8;
+      const int u = kt == 1 ? cfg.unroll : (cfg.stage_lds == 4 && cfg.unroll == 16 ? 16 : 8);
       // waves per window: waves_per_block, at most one per 16-byte chunk of
       // an entry, or (FP32, values in registers) one per right-hand side
       const bool regs = cfg.stage_lds == 4;  // no slices

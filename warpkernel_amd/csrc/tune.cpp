@@ -76,10 +76,18 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
           out.push_back(c);
           if (D.win_max > 0 && kt >= 2 && kt <= 8) {
             // windowed passes, values through slices or straight into registers
-            // (the launcher falls back where the window does not fit)
+            // (the latter also with 16-step batches at k-tiles 4 and 8, one or
+            // two waves per window); the launcher falls back where the window
+            // does not fit
             for (int st : {3, 4}) {
               c.stage_lds = st;
               out.push_back(c);
+            }
+            if (kt >= 4 && wpb <= 2) {
+              c.stage_lds = 4;
+              c.unroll = 16;
+              out.push_back(c);
+              c.unroll = 8;
             }
             c.stage_lds = 0;
           }
@@ -133,11 +141,12 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     // elasticity 32^3 and 64^3: FP64 k=4 17.3 / 119 us (stage 3: 23.7 /
     // 128), k=8 and 16 with k-tile 8 shared by two waves; at k=32 the
     // staged form with one wave per window stays ahead (111 / 817 us).  FP32
-    // (compensated) is bound by its arithmetic; on 32^3 a window per wave up
-    // to k = 8 (k=2 11.2, k=4 15.6, k=8 23.0 us, against 12.7 / 19.0 / 24.9
-    // with windows shared one right-hand side per wave), and from k = 16 four
-    // waves sharing a k-tile-8 window, two right-hand sides each (k=16 46.2
-    // against 49.4 with two waves, k=32 90.5 against 89.0).
+    // (compensated) is bound by its arithmetic; on 32^3 a window per wave
+    // below k = 16 (k=2 11.2, k=4 15.6, k=8 22.4 us, against 12.7 / 19.0 /
+    // 24.9 with windows shared one right-hand side per wave), and from k = 16
+    // two waves sharing a k-tile-8 window; from k = 8 16-step batches (two in
+    // flight: 32^3 k=32 84.3 us against 89.6 with 8-step ones, 64^3 k=16 304
+    // against 323).
     c.unroll = 8;
     c.xcd_remap = 1;
     if (dtype == WK_F64) {
@@ -159,7 +168,8 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     } else {
       c.stage_lds = 4;
       c.k_tile = std::min(8, pow2_at_most(k));
-      c.waves_per_block = k < 16 ? 1 : 4;
+      c.waves_per_block = k < 16 ? 1 : 2;
+      c.unroll = k < 8 ? 8 : 16;
     }
   } else if (k == 1 && D.W == 64 && mean_len >= 20.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
