@@ -239,7 +239,8 @@ typedef struct wk_launch {
                               right-hand sides (at most one wave per 16
                               bytes of a window entry; FP32 engines with
                               stage_lds 4: at most one per right-hand side) */
-  int32_t unroll;          /* slots in flight per lane: 4, 8 or 16 (k_tile 1) */
+  int32_t unroll;          /* slots in flight per lane: 4, 8 or 16 (k_tile 1;
+                              and 16 with stage_lds 4 at k_tiles 4 and 8) */
   int32_t k_tile;          /* right-hand sides per pass: 1, 2, 4, 8, 16 or 32 */
   int32_t xcd_remap;       /* 1: contiguous group ranges per XCD */
   int32_t nontemporal;     /* 1: streaming loads for values and indices */
