@@ -743,7 +743,9 @@ void launch_win(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size
 
 // k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8 and
 // two batches in flight (measured best of one, two and four), and 1, 2 or 4
-// waves sharing a window where each wave's share is whole 16-byte chunks
+// waves sharing a window where each wave's share is whole 16-byte chunks;
+// register-fed (D = 0) also with unroll 16 at k-tiles 4 and 8 (one or two
+// waves per window) and, for FP32, down to one right-hand side per wave
 template <class P, bool SPLIT, int OUT, int D>
 WinFn pick_win_d(int u, int kt, int wpw) {
   constexpr int xsz = (int)sizeof(typename P::X);
@@ -797,10 +799,9 @@ WinFn pick_win_d(int u, int kt, int wpw) {
   return nullptr;
 }
 
-// k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8 and
-// two batches in flight through the slices (measured best of one, two and
-// four) or, regs, loaded straight into registers; 1, 2 or 4 waves sharing a
-// window where each wave's share is whole 16-byte chunks
+// k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with two batches in
+// flight through the slices (measured best of one, two and four) or, regs,
+// loaded straight into registers (the shapes pick_win_d lists)
 template <class P, bool SPLIT, int OUT>
 WinFn pick_win_u(int u, int kt, int wpw, bool regs) {
   return regs ? pick_win_d<P, SPLIT, OUT, 0>(u, kt, wpw) : pick_win_d<P, SPLIT, OUT, 2>(u, kt, wpw);
