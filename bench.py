@@ -378,8 +378,8 @@ def main():
                     help="also run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the WPK SpMV path here has no multi-GPU form "
-              "(how it would shard is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the WPK SpMV path here has no multi-GPU form; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
