@@ -154,9 +154,11 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         c.stage_lds = 4;
         c.k_tile = pow2_at_most(k);
         c.waves_per_block = 1;
-      } else if (k <= 16 || mean_len < 40.0) {
+      } else if (k < 16 || (k == 16 && D.ngroups < 4096) || mean_len < 40.0) {
         // (short rows keep the register-fed form at every k: hex diffusion
-        // k=32 68 us against 86 staged)
+        // k=32 68 us against 86 staged; at k = 16 the staged form pays from a
+        // few thousand groups: elasticity 48^3 181 against 191 us, 64^3 421
+        // against 453, 32^3 62.8 against 60.8)
         c.stage_lds = 4;
         c.k_tile = 8;
         c.waves_per_block = 2;
