@@ -273,6 +273,19 @@ def extra_batch(torch, dev, steps, warmup, tune=True):
         "one_launch_per_matrix_ms": round(me, 4), "one_launch_per_matrix_GBs": round(tot / (me * 1e-3) / 1e9, 1),
     }
     g.close()
+    # k = 8 right-hand sides in one launch: x windows of 8 values per column
+    # (the default) against direct x loads
+    xs8 = [dev(fe.uniform_x(mm.ncols, 8, seed=7500 + i).reshape(mm.ncols, 8)) for i, mm in enumerate(ms)]
+    ys8 = [torch.empty((mm.nrows, 8), dtype=torch.float64, device="cuda") for mm in ms]
+    tot8 = sum(spmv_bytes(mm.nnz, mm.nrows, mm.ncols, 8) for mm in ms)
+    rec = out["batch_64_fe_matrices"]
+    for key, st in (("k8_one_launch", 5), ("k8_direct_loads", 0)):
+        b8 = batch(engs, stage_lds=st)
+        b8.bind(xs8, ys8)
+        _, m8 = time_launches(torch, b8.run, steps, warmup)
+        rec[key + "_ms"] = round(m8, 4)
+        rec[key + "_GBs"] = round(tot8 / (m8 * 1e-3) / 1e9, 1)
+        b8.close()
     return out
 
 

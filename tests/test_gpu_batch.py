@@ -51,8 +51,8 @@ def make_engines(ms, dtype="f64"):
 
 
 @pytest.mark.parametrize("k,stage,wpc", [(1, 0, 0), (1, 1, 0), (1, 2, 0), (1, 3, 0), (1, 5, 0),
-                                         (4, 3, 0), (4, 5, 0), (3, 0, 0), (1, 3, 8), (1, 5, 8),
-                                         (1, 0, 64), (4, 2, 16)])
+                                         (4, 3, 0), (4, 5, 0), (8, 5, 0), (12, 5, 0), (3, 0, 0),
+                                         (1, 3, 8), (1, 5, 8), (8, 5, 16), (1, 0, 64), (4, 2, 16)])
 def test_batch_of_64_matrices_bitexact(k, stage, wpc):
     ms = mix()
     say(f"[gpu] batch of {len(ms)} matrices, {sum(m.nnz for m in ms)} nnz, k={k}, stage={stage}, "
@@ -71,13 +71,15 @@ def test_batch_of_64_matrices_bitexact(k, stage, wpc):
         assert np.array_equal(ys[i].cpu().numpy(), expected(m, i, xs_h[i])), (i, m.name)
 
 
-def test_batch_fp32_compensated():
+@pytest.mark.parametrize("k", [1, 8])
+def test_batch_fp32_compensated(k):
     ms = mix()[:16]
-    say("[gpu] batch FP32")
+    say(f"[gpu] batch FP32 k={k}")
     engs = make_engines(ms, "f32")
-    xs_h = [fe.uniform_x(m.ncols, seed=7100 + i).astype(np.float32) for i, m in enumerate(ms)]
+    xs_h = [fe.uniform_x(m.ncols, k, seed=7100 + i).astype(np.float32) for i, m in enumerate(ms)]
+    shape = (lambda m: (m.nrows,)) if k == 1 else (lambda m: (m.nrows, k))
     for wpc, stage in ((4, 3), (0, 5)):
-        ys = [torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda") for m in ms]
+        ys = [torch.full(shape(m), float("nan"), dtype=torch.float64, device="cuda") for m in ms]
         b = batch(engs, waves_per_cu=wpc, stage_lds=stage)
         b.bind([dev(x) for x in xs_h], ys)
         b.run()

@@ -125,6 +125,19 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   const X *xb = static_cast<const X *>(d.x) + j0;
   Acc acc[KT];
   bool done = false;
+  if constexpr (STAGE == 5 && KT > 1) {
+    // several right-hand sides: the window holds KT values per column, values
+    // and positions go straight into registers (two 8-step batches
+    // alternating), as the single-matrix default
+    if (d.windowed) {
+      const uint32_t w0 = d.win_off[g], w1 = d.win_off[g + 1];
+      WinFill<X, KT> f;
+      f.cols(d.win_cols + w0, w1 - w0, lane);
+      f.gather(region, d.win_cols + w0, w1 - w0, xb, d.ldx, lane);
+      window_group_regs<P, U, KT, KT>(acc, vbase, d.win_idx + base, region, len, 0u, lane);
+      done = true;
+    }
+  }
   if constexpr (STAGE == 5 && KT == 1) {
     if (d.windowed && d.packed) {
       constexpr int NB = batch_nb<P>();
@@ -247,9 +260,9 @@ BatchFn pick_batch_kt(int kt, int stage) {
       if (stage == 2) return &launch_batch<P, 1, 2>;
       if (stage == 1) return &launch_batch<P, 1, 1>;
       return &launch_batch<P, 1, 0>;
-    case 2: return &launch_batch<P, 2, 0>;
-    case 4: return &launch_batch<P, 4, 0>;
-    case 8: return &launch_batch<P, 8, 0>;
+    case 2: return stage == 5 ? &launch_batch<P, 2, 5> : &launch_batch<P, 2, 0>;
+    case 4: return stage == 5 ? &launch_batch<P, 4, 5> : &launch_batch<P, 4, 0>;
+    case 8: return stage == 5 ? &launch_batch<P, 8, 5> : &launch_batch<P, 8, 0>;
   }
   return nullptr;
 }
@@ -311,6 +324,8 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
   return guarded([&] {
     if (!B || !x_devs || !y_devs || !ldx || !ldy) fail(WK_ERR_ARG, "NULL argument");
     if (k < 1 || k > 32) fail(WK_ERR_ARG, "k must be in [1, 32]");
+    // (k-tile 4 windows for k = 8 measured slower than k-tile 8: 0.737 against
+    // 0.659 ms on 64 FE matrices)
     const int32_t kt = k % 8 == 0 ? 8 : k % 4 == 0 ? 4 : k % 2 == 0 ? 2 : 1;
     const int32_t ktiles = k / kt;
     std::vector<BatchItem> items;
@@ -341,8 +356,12 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       d.staged = D.ngroups && mean_len >= 24.0;
       // x windows pay off where each entry serves enough slots (the rule of
       // the single-matrix default)
-      d.windowed = (B->stage == 3 || B->stage == 5) && kt == 1 && D.win_max > 0 &&
-                   D.win_total > 0 && (double)D.allocate_nz >= 4.5 * (double)D.win_total;
+      // (k-tiles above 1: stage 5 only, from 3 slots per entry as in the
+      // single-matrix default)
+      d.windowed = D.win_max > 0 && D.win_total > 0 &&
+                   (kt == 1 ? (B->stage == 3 || B->stage == 5) &&
+                                  (double)D.allocate_nz >= 4.5 * (double)D.win_total
+                            : B->stage == 5 && (double)D.allocate_nz >= 3.0 * (double)D.win_total);
       d.packed = B->stage == 5 && D.win_pk != nullptr;
       d.win_idx = D.win_idx;
       d.win_off = D.win_off;
@@ -350,8 +369,19 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       d.win_pk = D.win_pk;
       d.win_pk_off = D.win_pk_off;
       if (d.windowed) {
-        const int32_t xsz = E->dtype == WK_F64 ? 8 : 4;
-        win_bytes = std::max<int32_t>(win_bytes, (D.win_max * xsz + 15) / 16 * 16);
+        // a block's waves, each with this window and its slice, in 64 KB of
+        // LDS; a matrix whose window does not fit reads x directly
+        const bool f64 = E->dtype == WK_F64;
+        const int64_t win = ((int64_t)D.win_max * kt * (f64 ? 8 : 4) + 15) / 16 * 16;
+        const int64_t slice =
+            kt > 1 ? 16
+                   : B->stage == 5 ? (f64 ? batch_slice<PF64, 1, 8, 5>() : batch_slice<PF32C, 1, 8, 5>())
+                                   : (f64 ? batch_slice<PF64, 1, 8, 3>() : batch_slice<PF32C, 1, 8, 3>());
+        const int64_t wpb = B->stage == 5 ? batch_wpb<5>() : batch_wpb<3>();
+        if (wpb * (win + slice) <= 65536)
+          win_bytes = std::max<int32_t>(win_bytes, (int32_t)win);
+        else
+          d.windowed = 0;
       }
       if ((D.nrows && !y_devs[i]) || (E->ncols && !x_devs[i])) fail(WK_ERR_ARG, "NULL vector");
       for (int32_t g = 0; g < D.ngroups; g++) {
@@ -404,20 +434,6 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
     hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(), hipMemcpyHostToDevice), "bind");
     if (!wstart.empty())
       hip_check(hipMemcpy(B->d_wstart, wstart.data(), sizeof(int32_t) * wstart.size(), hipMemcpyHostToDevice), "bind");
-    // the windows and slices of a block's waves must fit one block's LDS
-    const bool f64 = B->dtype == WK_F64;
-    const size_t block_lds =
-        B->stage == 5 ? (size_t)batch_wpb<5>() * (win_bytes + (f64 ? batch_slice<PF64, 1, 8, 5>()
-                                                                     : batch_slice<PF32C, 1, 8, 5>()))
-                      : 4 * (size_t)(win_bytes + (f64 ? batch_slice<PF64, 1, 8, 3>()
-                                                        : batch_slice<PF32C, 1, 8, 3>()));
-    if (win_bytes && block_lds > 65536) {
-      for (BatchDesc &d : B->desc) d.windowed = 0;
-      win_bytes = 0;
-      hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(),
-                          hipMemcpyHostToDevice),
-                "bind");
-    }
     B->win_bytes = win_bytes;
     B->nitems = (int64_t)items.size();
     B->k = k;
