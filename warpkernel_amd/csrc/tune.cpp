@@ -113,8 +113,11 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
   // and from k=2 in FP32; the tiny 2D Laplacian only at k=32.
   const bool win_ok = D.W == 64 && D.win_max > 0 && D.win_total > 0;
   const double reuse = win_ok ? (double)D.allocate_nz / (double)D.win_total : 0.0;
+  // (k = 1 with packed positions from 3 slots per entry: hex diffusion 48^3,
+  // 3.4, forward and adjoint 7.16 us against 7.76 staged; P1 tets, 2.1, and
+  // the 2D Laplacian, 2.9, stay faster without windows)
   const bool windows =
-      win_ok && (k == 1 ? reuse >= 4.5
+      win_ok && (k == 1 ? reuse >= (D.win_pk ? 3.0 : 4.5)
                         : reuse >= 3.0 || (reuse >= 2.0 && (k >= 16 || (dtype != WK_F64 &&
                                                                          D.ngroups >= 1024))));
   if (k == 1 && windows) {
