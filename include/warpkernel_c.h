@@ -362,7 +362,8 @@ typedef struct wk_batch wk_batch;
  * 3 with packed window positions where a matrix has them, one wave per
  * block); staging applies to the matrices whose mean group length is at
  * least 24 and windows to those with windows whose entries serve at least 4.5
- * slots each on average; the others use direct loads.  With several
+ * slots each on average (3 with packed positions at stage_lds 5); the others
+ * use direct loads.  With several
  * right-hand sides, stage_lds 5 gives the matrices whose window entries serve
  * at least 3 slots windows of k-tile values per column (values and positions
  * straight into registers), the others direct loads; every other stage_lds

@@ -354,13 +354,14 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       // staging pays off for long rows only (it adds LDS traffic per batch)
       const double mean_len = D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * 64.0) : 0.0;
       d.staged = D.ngroups && mean_len >= 24.0;
-      // x windows pay off where each entry serves enough slots (the rule of
-      // the single-matrix default)
-      // (k-tiles above 1: stage 5 only, from 3 slots per entry as in the
-      // single-matrix default)
+      // x windows where each entry serves enough slots, as in the
+      // single-matrix default: one right-hand side from 3 slots per entry with
+      // packed positions (stage 5), 4.5 without; k-tiles above 1 (stage 5
+      // only) from 3
+      const bool pk5 = B->stage == 5 && D.win_pk != nullptr;
       d.windowed = D.win_max > 0 && D.win_total > 0 &&
                    (kt == 1 ? (B->stage == 3 || B->stage == 5) &&
-                                  (double)D.allocate_nz >= 4.5 * (double)D.win_total
+                                  (double)D.allocate_nz >= (pk5 ? 3.0 : 4.5) * (double)D.win_total
                             : B->stage == 5 && (double)D.allocate_nz >= 3.0 * (double)D.win_total);
       d.packed = B->stage == 5 && D.win_pk != nullptr;
       d.win_idx = D.win_idx;
