@@ -153,9 +153,13 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     c.unroll = 8;
     c.xcd_remap = 1;
     if (dtype == WK_F64) {
-      if (k < 8) {
+      if (k < 8 || (k < 16 && D.ngroups < 2048 && mean_len >= 40.0)) {
+        // (k = 8 on small layouts of long rows: k-tile 4, one wave per window;
+        // elasticity 32^3, 1685 groups, 30.8 us against 32.4 with k-tile 8
+        // over two waves; 40^3, 3231 groups, 56.6 against 55.4; hex diffusion
+        // 48^3, 27-entry rows, 17.6 against 16.9)
         c.stage_lds = 4;
-        c.k_tile = pow2_at_most(k);
+        c.k_tile = std::min(4, pow2_at_most(k));
         c.waves_per_block = 1;
       } else if (k < 16 || (k == 16 && D.ngroups < 4096) || mean_len < 40.0) {
         // (short rows keep the register-fed form at every k: hex diffusion
