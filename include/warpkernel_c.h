@@ -196,9 +196,22 @@ int wk_engine_update_values(wk_engine *engine, const void *vals_dev, void *strea
 /* The reference's drivers overwrite the public engine::device_colinds with the
  * Kr columns and restore them (wpk1.cu:160-197).  pcols has allocate_nz
  * entries in padded order; where = 0 host, 1 device.  Every entry must lie in
- * [0, ncols) (checked before the copy; WK_ERR_ARG otherwise). */
+ * [0, ncols) (checked before the copy; WK_ERR_ARG otherwise, nothing changed).
+ * The new columns feed wk_run, wk_residual and run_assembly's layout, and
+ * wk_run_x on engines without their own Kr columns (WK_ENGINE_KR engines
+ * keep running run_x on those).  The engine's x windows are built from its
+ * scanned columns: they are used while the columns equal those (an exact check
+ * on the device after each call), so restoring the original columns restores
+ * the windowed kernels.  Engines built with WK_ENGINE_ADJOINT refuse the call
+ * (WK_ERR_STATE): their transposed layout and dvals follow the scanned
+ * pattern.  Allocates a small check buffer and synchronises `stream`. */
 int wk_engine_set_colinds(wk_engine *engine, const int32_t *pcols, int32_t where, void *stream);
 int wk_engine_get_colinds(const wk_engine *engine, int32_t *pcols_host);
+
+/* Entries over all x windows the engine's forward launches use: 0 when it has
+ * none, or while its columns differ from the scanned ones (see
+ * wk_engine_set_colinds).  No reference counterpart. */
+int wk_engine_x_windows(const wk_engine *engine, int64_t *entries);
 
 /* structure::scan + process (warpkernel.hpp:309-365, 532-548) or
  * structure2::scan(.., threshold) + process (hpp:578-736, 749-769) with the
@@ -374,7 +387,11 @@ int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_
 /* Binds the vectors of every matrix: x_devs[i] ([ncols_i][ldx[i]]) and y_devs[i]
  * ([nrows_i][ldy[i]]) for k right-hand sides (1..32).  Host arrays of device
  * pointers; copied to the device here (synchronously), so bind outside graph
- * capture and re-bind when the vectors move. */
+ * capture and re-bind when the vectors move.  The batch's device arrays are
+ * allocated once, by wk_batch_create, and rewritten in place by every bind: a
+ * graph that captured wk_batch_run keeps working after a re-bind with the same
+ * k (it then reads the new vectors), but must be captured again after a bind
+ * that changes k (the launch shape and its arguments depend on k). */
 int wk_batch_bind(wk_batch *batch, const void *const *x_devs, double *const *y_devs, int32_t k,
                   const int64_t *ldx, const int64_t *ldy);
 

@@ -163,3 +163,37 @@ def test_batch_window_fit_per_matrix(k):
     for i, (r, c, v, nr, nc) in enumerate(mats):
         assert np.array_equal(ys[i].cpu().numpy(), O.csr_spmv(r, c, v, xs_h[i])), i
     b.close()
+
+
+@pytest.mark.parametrize("k", [1, 8])
+def test_graph_replay_after_rebind_same_k(k):
+    """A graph that captured wk_batch_run keeps working after re-binds with the
+    same k: the batch's device arrays are allocated once and rewritten in
+    place, so each replay reads the vectors of the latest bind."""
+    say(f"[gpu] graph replay after re-bind k={k}")
+    ms = mix()[:10]
+    engs = make_engines(ms)
+    shape = (lambda m: (m.ncols,)) if k == 1 else (lambda m: (m.ncols, k))
+    yshape = (lambda m: (m.nrows,)) if k == 1 else (lambda m: (m.nrows, k))
+    b = batch(engs)
+    xs_h = [fe.uniform_x(m.ncols, k, seed=7800 + i).reshape(shape(m)) for i, m in enumerate(ms)]
+    ys = [torch.full(yshape(m), float("nan"), dtype=torch.float64, device="cuda") for m in ms]
+    b.bind([dev(x) for x in xs_h], ys)
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with graph.capture(st) as g:
+        b.run(stream=st)
+    for rep in range(3):
+        # new vectors in new allocations, bound eagerly between replays
+        xs_h = [fe.uniform_x(m.ncols, k, seed=7900 + 100 * rep + i).reshape(shape(m))
+                for i, m in enumerate(ms)]
+        xs = [dev(x) for x in xs_h]
+        ys = [torch.full(yshape(m), float("nan"), dtype=torch.float64, device="cuda") for m in ms]
+        b.bind(xs, ys)
+        torch.cuda.synchronize()
+        g.launch(st)
+        st.synchronize()
+        for i, m in enumerate(ms):
+            assert np.array_equal(ys[i].cpu().numpy(), expected(m, i, xs_h[i])), (rep, i)
+    g.close()
+    b.close()

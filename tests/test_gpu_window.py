@@ -138,24 +138,59 @@ def test_fp32_window_matches_staged():
     eng.close()
 
 
-def test_swapped_columns_drop_the_windows():
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_swapped_columns_drop_and_restore_the_windows(where):
     """The drivers' device_colinds swap (wpk1.cu:175) replaces the columns the
-    windows were built from: launches run the staged kernel from then on."""
-    say("[gpu] window after device_colinds swap")
-    m = fe.CONFIGS["q1_laplace_2d_128"]()
+    windows were built from: launches run the staged kernel while they differ,
+    and the windows return when the original columns are restored
+    (wpk1.cu:196), from a host array or a device tensor."""
+    say(f"[gpu] window after device_colinds swap ({where})")
+    m = fe.CONFIGS["hex_elastic_3d_32"]()
     x = fe.uniform_x(m.ncols, seed=9)
     ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
     s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
     eng = engine(s, m.vals, m.colind)
+    built = eng.x_window_entries()
+    assert built > 0
+    put = (lambda a: a) if where == "host" else dev
     restore = eng.device_colinds
-    eng.device_colinds = s.reorder_columns_coalesced(m.colind)
-    yx = nan_vec(m.nrows)
-    eng.launch_x(dev(s.reorder_x(x)), yx, cfg=dict(WIN, unroll=8, waves_per_block=4))
-    assert np.array_equal(host(yx), ref[s.row_map])
-    eng.device_colinds = restore
-    y = nan_vec(m.nrows)
-    eng.launch(dev(x), y, cfg=dict(WIN, unroll=8, waves_per_block=4))
-    assert np.array_equal(host(y), ref)
+    reorder_cols = s.reorder_columns_coalesced(m.colind)
+    for st in (3, 5):
+        eng.device_colinds = put(reorder_cols)
+        assert eng.x_window_entries() == 0
+        yx = nan_vec(m.nrows)
+        eng.launch_x(dev(s.reorder_x(x)), yx, cfg=dict(WIN, unroll=8, waves_per_block=4, stage_lds=st))
+        assert np.array_equal(host(yx), ref[s.row_map])
+        eng.device_colinds = put(restore)
+        assert eng.x_window_entries() == built
+        y = nan_vec(m.nrows)
+        eng.launch(dev(x), y, cfg=dict(WIN, unroll=16, waves_per_block=1, stage_lds=st))
+        assert np.array_equal(host(y), ref)
+    # columns that differ in one slot only keep the windows off
+    one = restore.copy()
+    one[len(one) // 2] = (one[len(one) // 2] + 1) % m.ncols
+    eng.device_colinds = put(one)
+    assert eng.x_window_entries() == 0
+    eng.device_colinds = put(restore)
+    assert eng.x_window_entries() == built
+    # out-of-range columns are refused and change nothing
+    bad = restore.copy()
+    bad[7] = m.ncols
+    with pytest.raises(capi.WKError):
+        eng.device_colinds = put(bad)
+    assert np.array_equal(eng.device_colinds, restore) and eng.x_window_entries() == built
+    eng.close()
+
+
+def test_adjoint_engine_refuses_column_swap():
+    say("[gpu] adjoint engine refuses device_colinds")
+    m = fe.hex_laplace_3d(8)
+    s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind, adjoint=True)
+    cols = eng.device_colinds
+    with pytest.raises(capi.WKError) as e:
+        eng.device_colinds = cols
+    assert e.value.code == capi.WK_ERR_STATE
     eng.close()
 
 

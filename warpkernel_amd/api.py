@@ -422,6 +422,13 @@ class engine:
         else:
             check(lib().wk_engine_set_colinds(self._h, _dev_ptr(pcols), 1, _stream(None)))
 
+    def x_window_entries(self) -> int:
+        """Entries over all x windows the forward launches use (0: none, or the
+        columns were replaced by device_colinds)."""
+        n = C.c_int64()
+        check(lib().wk_engine_x_windows(self._h, C.byref(n)))
+        return n.value
+
     # -- verification (host) ------------------------------------------------
     def verify(self, orig, comp) -> bool:
         """engine::verify (hpp:932-943): relative 1e-5 where |orig| > 1e-14,

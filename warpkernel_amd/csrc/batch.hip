@@ -62,9 +62,12 @@ struct wk_batch {
   int32_t stage = 2;
   std::vector<wk::BatchDesc> desc;
   wk::BatchDesc *d_desc = nullptr;
+  // device arrays of a fixed size, allocated at creation and freed only at
+  // destruction, so a graph captured after a bind stays valid across re-binds
+  // with the same k (they rewrite the arrays in place)
   wk::BatchItem *d_items = nullptr;
   int32_t *d_wstart = nullptr;
-  int64_t nitems = 0;
+  int64_t nitems = 0;       // work items: every group and fill chunk of every engine
   int32_t win_bytes = 0;  // LDS window per wave (largest windowed matrix), 16-byte multiple
 };
 
@@ -308,6 +311,14 @@ int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_
       B->nwaves = waves_per_cu ? cus * waves_per_cu : 0;
       B->desc.resize(count);
       hip_check(hipMalloc(&B->d_desc, sizeof(BatchDesc) * count), "hipMalloc");
+      // the work list has one item per group and per 64 rows without nonzeros
+      // of every engine, whatever the bound vectors and k
+      int64_t nitems = 0;
+      for (const wk_engine *E : B->engines)
+        nitems += (int64_t)E->fwd.ngroups + (E->fwd.nrows - E->fwd.nznrows + 63) / 64;
+      if (nitems > (int64_t)INT32_MAX) fail(WK_ERR_LIMIT, "too many work items");
+      B->nitems = nitems;
+      if (nitems) hip_check(hipMalloc(&B->d_items, sizeof(BatchItem) * (size_t)nitems), "hipMalloc");
       if (B->nwaves) {
         hip_check(hipMalloc(&B->d_wstart, sizeof(int32_t) * (B->nwaves + 1)), "hipMalloc");
       }
@@ -426,17 +437,13 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
       }
       while (w < B->nwaves) wstart[++w] = (int32_t)items.size();
     }
-    (void)hipFree(B->d_items);
-    B->d_items = nullptr;
-    if (!items.empty())
-      hip_check(hipMalloc(&B->d_items, sizeof(BatchItem) * items.size()), "hipMalloc");
+    if ((int64_t)items.size() != B->nitems) fail(WK_ERR_STATE, "work list changed size since creation");
     if (!items.empty())
       hip_check(hipMemcpy(B->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice), "bind");
     hip_check(hipMemcpy(B->d_desc, B->desc.data(), sizeof(BatchDesc) * B->desc.size(), hipMemcpyHostToDevice), "bind");
     if (!wstart.empty())
       hip_check(hipMemcpy(B->d_wstart, wstart.data(), sizeof(int32_t) * wstart.size(), hipMemcpyHostToDevice), "bind");
     B->win_bytes = win_bytes;
-    B->nitems = (int64_t)items.size();
     B->k = k;
     B->kt = kt;
   });

@@ -368,24 +368,24 @@ int wk_engine_set_colinds(wk_engine *E, const int32_t *pcols, int32_t where, voi
   return guarded([&] {
     check_engine(E);
     if (!pcols && E->fwd.allocate_nz) fail(WK_ERR_ARG, "pcols is NULL");
-    const size_t bytes = sizeof(int32_t) * (size_t)E->fwd.allocate_nz;
-    if (!bytes) return;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    // the kernels gather x[col]: reject out-of-range columns before they land
-    if (where)
-      check_index_range_device(pcols, E->fwd.allocate_nz, E->ncols, s);
-    else
-      for (int64_t i = 0; i < E->fwd.allocate_nz; i++)
-        if ((uint32_t)pcols[i] >= (uint32_t)E->ncols)
-          fail(WK_ERR_ARG, "padded column out of range at slot " + std::to_string(i));
-    hip_check(hipMemcpyAsync(E->fwd.cols, pcols, bytes,
-                             where ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s),
-              "set_colinds");
-    // the x windows describe the old columns: launches fall back to staging
-    // without windows from here on
-    E->fwd.win_max = 0;
-    E->fwd.win_total = 0;
-    if (!where) hip_check(hipStreamSynchronize(s), "set_colinds");
+    if (!E->fwd.allocate_nz) return;
+    // device memory for the check kernels' flags, freed on every path
+    struct Flags {
+      int32_t *p = nullptr;
+      ~Flags() {
+        if (p) (void)hipFree(p);
+      }
+    } flags;
+    hip_check(hipMalloc(reinterpret_cast<void **>(&flags.p), sizeof(int32_t) * kColFlags), "hipMalloc");
+    swap_colinds(E, pcols, where, static_cast<hipStream_t>(stream), flags.p);
+  });
+}
+
+int wk_engine_x_windows(const wk_engine *E, int64_t *entries) {
+  return guarded([&] {
+    check_engine(E);
+    if (!entries) fail(WK_ERR_ARG, "entries is NULL");
+    *entries = E->fwd.win_max > 0 ? E->fwd.win_total : 0;
   });
 }
 
@@ -521,24 +521,28 @@ int wk_time_run(wk_engine *E, const wk_launch *cfg, int32_t which, const void *x
   return guarded([&] {
     check_engine(E);
     if (!seconds) fail(WK_ERR_ARG, "seconds is NULL");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    hipEvent_t e0, e1;
-    hip_check(hipEventCreate(&e0), "event");
-    hip_check(hipEventCreate(&e1), "event");
-    int rc;
-    hip_check(hipEventRecord(e0, s), "event");
+    if (which < 0 || which > 2) fail(WK_ERR_ARG, "which must be 0 (run), 1 (run_x) or 2 (run_assembly)");
     if (which == 2 && k != 1) fail(WK_ERR_ARG, "run_assembly has one output column");
-    rc = which == 0   ? wk_run(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
-         : which == 1 ? wk_run_x(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
-                      : wk_run_assembly(E, cfg, y_dev, stream);
-    const hipError_t r1 = hipEventRecord(e1, s);
-    const hipError_t r2 = r1 == hipSuccess ? hipEventSynchronize(e1) : r1;
-    float ms = 0;
-    const hipError_t r3 = r2 == hipSuccess ? hipEventElapsedTime(&ms, e0, e1) : r2;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the two events are destroyed on every path out of this scope
+    struct Events {
+      hipEvent_t e[2] = {nullptr, nullptr};
+      ~Events() {
+        for (hipEvent_t ev : e)
+          if (ev) (void)hipEventDestroy(ev);
+      }
+    } ev;
+    hip_check(hipEventCreate(&ev.e[0]), "event");
+    hip_check(hipEventCreate(&ev.e[1]), "event");
+    hip_check(hipEventRecord(ev.e[0], s), "event");
+    const int rc = which == 0   ? wk_run(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
+                   : which == 1 ? wk_run_x(E, cfg, x_dev, y_dev, k, ldx, ldy, stream)
+                                : wk_run_assembly(E, cfg, y_dev, stream);
     if (rc != WK_OK) fail(rc, g_last_error);
-    hip_check(r3, "timing events");
+    hip_check(hipEventRecord(ev.e[1], s), "event");
+    hip_check(hipEventSynchronize(ev.e[1]), "event");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "timing events");
     *seconds = ms * 1e-3f;
   });
 }

@@ -176,23 +176,6 @@ __global__ __launch_bounds__(256) void fill_kernel(
   }
 }
 
-// flags[b] = 1 if block b saw an index outside [0, bound)
-__global__ __launch_bounds__(256) void range_check_kernel(const int32_t *__restrict__ a, int64_t n,
-                                                          int32_t bound, int32_t *__restrict__ flags) {
-  __shared__ int32_t red[256];
-  int32_t bad = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    bad |= (uint32_t)a[i] >= (uint32_t)bound;
-  red[threadIdx.x] = bad;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] |= red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) flags[blockIdx.x] = red[0];
-}
-
 // ---------------------------------------------------------------- x windows
 // The windows of build_windows (scan.cpp), built on the device: one block per
 // group sorts the group's slot keys (the entry's column, -1 for a pad) in LDS,
@@ -514,23 +497,6 @@ void device_scan(wk_engine *E, int32_t variant, int32_t threshold, int32_t max_t
 }
 
 }  // namespace
-
-void check_index_range_device(const int32_t *a, int64_t n, int32_t bound, hipStream_t s) {
-  if (n <= 0) return;
-  const int32_t nb = (int32_t)std::min<int64_t>((n + 4095) / 4096, 1024);
-  int32_t *flags = nullptr;
-  hip_check(hipMalloc(reinterpret_cast<void **>(&flags), sizeof(int32_t) * nb), "hipMalloc");
-  hipLaunchKernelGGL(range_check_kernel, dim3(nb), dim3(256), 0, s, a, n, bound, flags);
-  hipError_t e = hipGetLastError();
-  std::vector<int32_t> h((size_t)nb, 0);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h.data(), flags, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(flags);
-  hip_check(e, "index range check");
-  for (int32_t v : h)
-    if (v) fail(WK_ERR_ARG, "index out of range [0, " + std::to_string(bound) + ")");
-}
 
 }  // namespace wk
 

@@ -560,7 +560,7 @@ constexpr int kBwdTiles = 2;
 
 // dx = A^T dy and dvals in one launch, for layouts of short groups: the
 // adjoint blocks (the direct body, one right-hand side, no LDS) and the dvals
-// tiles (8 KB of LDS) are spread evenly over the grid, so one kind's
+// tiles (kBwdTiles x 8 KB = 16 KB of LDS) are spread evenly over the grid, so one kind's
 // latency-bound phases overlap the other's streaming and the launch and tail
 // costs are paid once.  Block b is dvals tile floor(b*ndv/T) when that count
 // steps at b, else adjoint block b - floor(b*ndv/T), T = nspmv + ndv.

@@ -93,6 +93,10 @@ struct DevLayout {
   int32_t *win_cols_kr = nullptr;  // the same columns in Kr numbering (row_map_inv)
   int32_t win_max = 0;             // largest window; 0 when the layout has none
   int64_t win_total = 0;           // win_off[ngroups]: entries over all windows
+  // the values above as built, kept while a column swap turns the windows off
+  // (they describe the engine's own columns only)
+  int32_t win_max_built = 0;
+  int64_t win_total_built = 0;
   // the same positions packed 10 bits each, three per 32-bit word, when every
   // window has at most 1024 entries: per group ceil(len/12) blocks of 64 lanes
   // x 16 bytes, lane l's block holding its 12 positions of those 12 steps
@@ -170,9 +174,16 @@ void launch_stream_read(const void *p, int64_t bytes, uint32_t *sink, hipStream_
 // packed form (width 64, win_max <= kPackedWindow); otherwise leaves them null.
 void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream);
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
-// Fails with WK_ERR_ARG unless every a[i], i < n, lies in [0, bound); device
-// array, synchronises the stream.
-void check_index_range_device(const int32_t *a, int64_t n, int32_t bound, hipStream_t stream);
+// Flags (int32) the column swap's check kernels write, one per block.
+constexpr int32_t kColFlags = 1024;
+// Replaces the forward layout's padded columns with pcols (allocate_nz
+// entries; where = 0 host, 1 device) after checking every entry lies in
+// [0, ncols) (WK_ERR_ARG otherwise, nothing changed); the x windows stay in use
+// exactly while the columns equal the ones they were built from.  flags:
+// kColFlags int32 of device memory for the checks.  Engines with the adjoint
+// layout are refused (WK_ERR_STATE).  Synchronises the stream.
+void swap_colinds(wk_engine *E, const int32_t *pcols, int32_t where, hipStream_t stream,
+                  int32_t *flags);
 // Row sums of the padded values (run_assembly); cfg supplies waves_per_block
 // and xcd_remap.
 void launch_rowsum(const DevLayout &D, int32_t dtype, const wk_launch &cfg, double *y,
