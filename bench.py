@@ -697,8 +697,13 @@ def main():
     if args.write_traffic:
         # the launch shape comes from the JSON line the counter runs printed
         wl, fd, wd, case_json = args.write_traffic
+        launch = None
         with open(case_json) as f:
-            launch = json.loads(f.read().strip().splitlines()[-1])["launch"]
+            for ln in f:  # the case's line among the profiler's own output
+                if ln.startswith("{") and '"case"' in ln:
+                    launch = json.loads(ln)["launch"]
+        if launch is None:
+            raise SystemExit(f"no case line in {case_json}")
         write_traffic(wl, fd, wd, launch)
         return
 

@@ -137,7 +137,30 @@ int wk_layout_reorder_columns_rowsort(wk_layout *layout, const int32_t *rowptr_h
 int wk_layout_reorder_values_rowsort(const wk_layout *layout, const double *vals_host,
                                      double *pvals_host);
 
+/* A layout from its arrays: the state structure::store / storemap /
+ * storereorder and structure2::store_param leave in a structure's public
+ * fields (warpkernel.hpp:288-307, 567-576), together with the fields the
+ * caller sets (warp_offsets, max_nz_rows, nwarps, nznrows, allocate_nz) and the
+ * CSR row offsets (nrows + 1).  row_map_inv is derived; min_nz / max_nz
+ * (variant 2) are computed from rowptr.  Every invariant the kernels index by
+ * is checked, as for a loaded layout file (WK_ERR_CSR).  threads_per_row and
+ * row_offset_warp are read for variant 2 only. */
+int wk_layout_from_arrays(int32_t variant, int32_t group_width, int32_t threshold, int32_t max_tpr,
+                          int32_t nrows, int32_t ncols, int32_t nznrows, int32_t nwarps, int64_t nz,
+                          int64_t allocate_nz, const int32_t *rowptr_host, const int32_t *row_map_host,
+                          const uint32_t *reorder_rows_host, const int64_t *warp_offsets_host,
+                          const uint32_t *max_nz_rows_host, const uint32_t *threads_per_row_host,
+                          const uint32_t *row_offset_warp_host, wk_layout **out);
+
 void wk_layout_destroy(wk_layout *layout);
+
+/* thrust::scatter as the reference's reorder_columns_coalesced_GPU and
+ * reorder_values_coalesced_GPU use it (warpkernel.hpp:459-502): dst[index[i]]
+ * = src[i] for i < n, device arrays of 4- or 8-byte elements, asynchronous on
+ * `stream`.  Indices at or beyond dst_count are skipped (the reference does
+ * not check them). */
+int wk_scatter(const void *src_dev, const uint32_t *index_dev, void *dst_dev, int64_t n,
+               int64_t dst_count, int32_t elem_bytes, void *stream);
 
 /* ------------------------------------------------------------------- files */
 
@@ -277,7 +300,16 @@ typedef struct wk_launch {
                               with window positions packed 10 bits each
                               where every window has at most 1024 entries
                               (unroll 8: 12-step batches, 16: 24-step),
-                              otherwise as 3; otherwise ignored */
+                              otherwise as 3; 6 (several right-hand
+                              sides, engines whose windows have packed
+                              positions and whose groups have at most 84
+                              steps): each wave loads its group's values
+                              and positions into registers once and makes
+                              one pass per tile of 4 (FP64) or 8 (FP32)
+                              right-hand sides through a window of that
+                              many values per column; the rest of k (fewer
+                              than one tile), and engines it does not fit,
+                              run as 4; otherwise ignored */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);

@@ -27,6 +27,265 @@ static void expect(bool ok, const char *what) {
   }
 }
 
+// The drivers' environment: their cusp / thrust names over this library's
+// containers (the drivers define ValueType and friends after their includes,
+// wpk1.cu:17-21).
+namespace thrust {
+template <class T>
+T *raw_pointer_cast(T *p) {
+  return p;
+}
+}  // namespace thrust
+namespace cusp {
+struct host_memory {};
+struct device_memory {};
+template <class T, class Space>
+using array1d = typename std::conditional<std::is_same<Space, device_memory>::value,
+                                          warpkernel::device_vector<T>, std::vector<T>>::type;
+template <class I, class V, class Space>
+using csr_matrix = warpkernel::csr_matrix;
+}  // namespace cusp
+#define ValueType double
+#define IndexType int
+#define DeviceSpace cusp::device_memory
+#define CPUSpace cusp::host_memory
+
+template <class T>
+static std::vector<T> host_of(const warpkernel::device_vector<T> &d) {
+  return std::vector<T>(d);
+}
+
+// wpk1.cu:118-135, wpk2.cu:136-165 and serialize.cu:76-98, their lines kept as
+// the drivers have them, then every array checked against the oracle.
+static void drivers_verbatim(const std::vector<int> &rowptr, const std::vector<int> &colind,
+                             const std::vector<double> &vals, const std::vector<double> &xin,
+                             const std::vector<double> &y, uint M, uint N, uint nz) {
+  cusp::csr_matrix<IndexType, ValueType, CPUSpace> A;
+  A.num_rows = M;
+  A.num_cols = N;
+  A.num_entries = nz;
+  A.row_offsets = rowptr;
+  A.column_indices = colind;
+  A.values = vals;
+  cusp::array1d<ValueType, CPUSpace> x = xin;
+  wko_layout *L1 = wko_scan1(64, (int32_t)M, nz, rowptr.data());
+  std::vector<double> opv(L1->allocate_nz);
+  std::vector<int32_t> opc(L1->allocate_nz);
+  wko_process(L1, vals.data(), colind.data(), opv.data(), opc.data());
+
+  // wpk1.cu:120-135
+  {
+    cusp::array1d<ValueType, DeviceSpace> dx = x;
+    warpkernel::structure kernel1;
+    kernel1.scan(nz, M, A);
+
+    cusp::array1d<IndexType, CPUSpace> reorder_cols = A.column_indices;
+    kernel1.reorder_columns_coalesced(reorder_cols);
+
+    cusp::array1d<ValueType, DeviceSpace> dreordered_x;
+    kernel1.reorder_x(x, dreordered_x);
+
+    warpkernel::engine<ValueType, IndexType> eng(kernel1,
+						 &(A.values[0]),
+						 &(A.column_indices[0]));
+    uint warps_per_block = 4;
+    uint nblocks = (kernel1.nwarps + warps_per_block - 1) / warps_per_block;
+    uint blocksize = warps_per_block * 64;
+    cusp::array1d<ValueType, DeviceSpace> dy(N, 0);
+    eng.run<true>(nblocks, blocksize, dx.data(), dy.data());
+    expect(host_of(dy) == y, "wpk1.cu flow: scan(nz, M, A), engine, run bit-exact");
+    expect(std::vector<uint>(L1->reorder_rows, L1->reorder_rows + nz) == kernel1.reorder_rows,
+           "wpk1.cu flow: layout matches the oracle");
+    // warp_offsets into the reference's std::vector<int>
+    std::vector<int> wo = kernel1.warp_offsets;
+    expect(std::vector<int64_t>(wo.begin(), wo.end()) ==
+               std::vector<int64_t>(L1->warp_offsets, L1->warp_offsets + L1->nwarps + 1),
+           "warp_offsets as std::vector<int>");
+    // the engine's public device arrays (hpp:782-788)
+    cusp::array1d<IndexType, DeviceSpace> restore_col = eng.device_colinds;
+    expect(host_of(restore_col) == opc, "engine.device_colinds read");
+    std::vector<double> dv = eng.device_values;
+    expect(dv == opv, "engine.device_values read");
+    std::vector<int> rm = eng.device_row_map, dwo = eng.device_warp_offsets;
+    std::vector<uint> mx = eng.device_max_nz_per_row;
+    expect(rm == kernel1.row_map && dwo == wo && mx == kernel1.max_nz_rows, "engine layout arrays read");
+    // new values through the public array (padded order) then run
+    for (auto &v : dv) v *= 2.0;
+    eng.device_values = dv;
+    cusp::array1d<ValueType, DeviceSpace> dy2(N, 0);
+    eng.run<true>(nblocks, blocksize, dx.data(), dy2.data());
+    std::vector<double> y2 = host_of(dy2);
+    bool twice = true;
+    for (uint i = 0; i < M; i++) twice = twice && y2[i] == 2.0 * y[i];
+    expect(twice, "engine.device_values = 2 A: y doubles exactly");
+    try {
+      std::vector<int> bad_map = kernel1.row_map;
+      std::swap(bad_map[0], bad_map[1]);
+      eng.device_row_map = bad_map;
+      expect(false, "a layout array other than the structure's is refused");
+    } catch (const warpkernel::error &) {
+    }
+  }
+
+  // serialize.cu:79-98: process into five device arrays
+  {
+    warpkernel::structure kernel1;
+    kernel1.scan(nz, N, A);
+
+    // allocate arrays
+    cusp::array1d<ValueType, DeviceSpace> device_values; // nz
+    cusp::array1d<IndexType, DeviceSpace> device_colinds; // nz
+    cusp::array1d<IndexType, DeviceSpace> device_row_map; // nrows
+    cusp::array1d<uint, DeviceSpace> device_max_nz_per_row; // nwarps
+    cusp::array1d<IndexType, DeviceSpace> device_warp_offsets; // nwarps
+    cusp::array1d<ValueType, DeviceSpace> dx = x;
+    cusp::array1d<ValueType, DeviceSpace> dy(N);
+
+    kernel1.process(thrust::raw_pointer_cast(&A.values[0]),
+    		    thrust::raw_pointer_cast(&A.column_indices[0]),
+    		    device_values, device_colinds,
+    		    device_row_map, device_max_nz_per_row,
+    		    device_warp_offsets);
+
+    expect(host_of(device_values) == opv && host_of(device_colinds) == opc,
+           "serialize.cu flow: padded arrays match the oracle's process");
+    expect(host_of(device_row_map) == std::vector<int>(L1->row_map, L1->row_map + M),
+           "serialize.cu flow: row map");
+    expect(host_of(device_max_nz_per_row) == std::vector<uint>(L1->max_nz_rows, L1->max_nz_rows + L1->nwarps),
+           "serialize.cu flow: group lengths");
+    std::vector<int> wo = host_of(device_warp_offsets);
+    expect(std::vector<int64_t>(wo.begin(), wo.end()) ==
+               std::vector<int64_t>(L1->warp_offsets, L1->warp_offsets + L1->nwarps + 1),
+           "serialize.cu flow: group offsets as int");
+
+    // engine(structure&) (hpp:790-792), filled by process, then run
+    warpkernel::engine<ValueType, IndexType> eng(kernel1);
+    kernel1.process(thrust::raw_pointer_cast(&A.values[0]),
+    		    thrust::raw_pointer_cast(&A.column_indices[0]),
+    		    eng.device_values, eng.device_colinds,
+    		    eng.device_row_map, eng.device_max_nz_per_row,
+    		    eng.device_warp_offsets);
+    eng.run<true>((kernel1.nwarps + 3) / 4, 256, dx.data(), dy.data());
+    expect(host_of(dy) == y, "engine(structure&) + process into its arrays: run bit-exact");
+
+    // reorder_columns (hpp:401-411) and the _GPU scatters (hpp:459-502)
+    cusp::array1d<IndexType, CPUSpace> kr = A.column_indices;
+    kernel1.reorder_columns(kr);
+    bool perm = true;
+    for (uint i = 0; i < nz; i++) perm = perm && kr[i] == kernel1.row_map_inv[A.column_indices[i]];
+    expect(perm, "reorder_columns");
+    cusp::array1d<IndexType, CPUSpace> krc = A.column_indices;
+    kernel1.reorder_columns_coalesced(krc);
+    cusp::array1d<IndexType, DeviceSpace> dkr = kr, dkrc(kernel1.allocate_nz, 0);
+    cusp::array1d<uint, DeviceSpace> drr = kernel1.reorder_rows;
+    kernel1.reorder_columns_coalesced_GPU(dkr, dkrc, drr);
+    expect(host_of(dkrc) == krc, "reorder_columns_coalesced_GPU equals the host form");
+    cusp::array1d<ValueType, DeviceSpace> dval1, dval2(kernel1.allocate_nz, 0.0);
+    kernel1.reorder_values_coalesced(A.values, dval1);
+    cusp::array1d<ValueType, DeviceSpace> dvals = A.values;
+    kernel1.reorder_values_coalesced_GPU(dvals, dval2, drr);
+    expect(host_of(dval1) == opv && host_of(dval2) == opv, "reorder_values_coalesced (host and _GPU)");
+
+    // store / storemap / storereorder (hpp:288-307): a structure rebuilt from
+    // its saved maps plus the group fields, then an engine on it
+    warpkernel::structure restored(kernel1.nrows, kernel1.nz, kernel1.allocate_nz);
+    std::vector<int> map = kernel1.row_map;
+    std::vector<uint> rr = kernel1.reorder_rows;
+    restored.store(map, rr);
+    restored.warp_offsets = kernel1.warp_offsets;
+    restored.max_nz_rows = kernel1.max_nz_rows;
+    restored.nwarps = kernel1.nwarps;
+    restored.nznrows = kernel1.nznrows;
+    warpkernel::engine<ValueType, IndexType> eng2(restored, &(A.values[0]), &(A.column_indices[0]));
+    cusp::array1d<ValueType, DeviceSpace> dy3(N, 0);
+    eng2.run<true>(1, 256, dx.data(), dy3.data());
+    expect(host_of(dy3) == y, "structure from store(): run bit-exact");
+    // a damaged stored map is refused when used
+    warpkernel::structure broken(kernel1.nrows, kernel1.nz, kernel1.allocate_nz);
+    std::vector<uint> rr_bad = rr;
+    rr_bad[0] = kernel1.allocate_nz + 5;
+    broken.store(map, rr_bad);
+    broken.warp_offsets = kernel1.warp_offsets;
+    broken.max_nz_rows = kernel1.max_nz_rows;
+    broken.nwarps = kernel1.nwarps;
+    broken.nznrows = kernel1.nznrows;
+    try {
+      warpkernel::engine<ValueType, IndexType> e3(broken, &(A.values[0]), &(A.column_indices[0]));
+      expect(false, "store() with a slot outside the padded arrays is refused");
+    } catch (const warpkernel::error &) {
+    }
+  }
+
+  // wpk2.cu:139-165 with structure2::process into seven arrays (hpp:749-769)
+  {
+    cusp::array1d<ValueType, DeviceSpace> dx = x;
+
+    warpkernel::structure2 kernel2;
+    kernel2.scan(nz, N, A, (int)N); // use maximum threshold
+
+    int max_nz = kernel2.max_nz;
+    int min_nz = kernel2.min_nz;
+    int inc = (max_nz-min_nz)*0.1 > 0 ? (max_nz-min_nz)*0.1 : 1;
+    for (int threshold = min_nz; threshold <= max_nz; threshold+=inc) {
+	kernel2.scan(nz, N, A, threshold); // use maximum threshold
+
+	cusp::array1d<IndexType, CPUSpace> reorder_cols = A.column_indices;
+	kernel2.reorder_columns_coalesced(reorder_cols);
+
+	cusp::array1d<ValueType, DeviceSpace> dreordered_x;
+	kernel2.reorder_x(x, dreordered_x);
+
+	warpkernel::engine<ValueType, IndexType> eng(kernel2,
+						     &(A.values[0]),
+						     &(A.column_indices[0]));
+
+      wko_layout *L2 = wko_scan2(64, threshold, 32, (int32_t)N, nz, rowptr.data());
+      std::vector<double> pv(L2->allocate_nz), yw(M);
+      std::vector<int32_t> pc(L2->allocate_nz);
+      wko_process(L2, vals.data(), colind.data(), pv.data(), pc.data());
+      wko_wpk_spmv(L2, pv.data(), pc.data(), xin.data(), yw.data(), 1, 1, 1, 0);
+      cusp::array1d<ValueType, DeviceSpace> dy(N, 0);
+      eng.run<true>((kernel2.nwarps + 3) / 4, 256, dx.data(), dy.data());
+      expect(host_of(dy) == yw, "wpk2.cu flow: scan(nz, N, A, threshold), run bit-exact");
+
+      cusp::array1d<ValueType, DeviceSpace> device_values;
+      cusp::array1d<IndexType, DeviceSpace> device_colinds, device_row_map, device_warp_offsets;
+      cusp::array1d<uint, DeviceSpace> device_max_nz_per_row, device_threads_per_row,
+          device_row_offset_warp;
+      kernel2.process(&(A.values[0]), &(A.column_indices[0]), device_values, device_colinds,
+                      device_row_map, device_max_nz_per_row, device_warp_offsets,
+                      device_threads_per_row, device_row_offset_warp);
+      expect(host_of(device_values) == pv && host_of(device_colinds) == pc,
+             "structure2::process: padded arrays match the oracle");
+      expect(host_of(device_threads_per_row) ==
+                     std::vector<uint>(L2->threads_per_row, L2->threads_per_row + L2->nwarps) &&
+                 host_of(device_row_offset_warp) ==
+                     std::vector<uint>(L2->row_offset_warp, L2->row_offset_warp + L2->nwarps + 1),
+             "structure2::process: threads_per_row and row_offset_warp");
+      std::vector<uint> tpr = eng.device_threads_per_row, row_off = eng.device_row_offset_warp;
+      expect(tpr == kernel2.threads_per_row && row_off == kernel2.row_offset_warp,
+             "structure2 engine arrays read");
+
+      // store / store_param (hpp:567-576)
+      warpkernel::structure2 restored(kernel2.nrows, kernel2.nz, kernel2.allocate_nz, threshold);
+      std::vector<int> map = kernel2.row_map;
+      std::vector<uint> rr = kernel2.reorder_rows, roff = kernel2.row_offset_warp,
+                        tp = kernel2.threads_per_row;
+      restored.store(map, rr, roff, tp);
+      restored.warp_offsets = kernel2.warp_offsets;
+      restored.max_nz_rows = kernel2.max_nz_rows;
+      restored.nwarps = kernel2.nwarps;
+      restored.nznrows = kernel2.nznrows;
+      warpkernel::engine<ValueType, IndexType> e2(restored, &(A.values[0]), &(A.column_indices[0]));
+      cusp::array1d<ValueType, DeviceSpace> dy2(N, 0);
+      e2.run<true>(1, 256, dx.data(), dy2.data());
+      expect(host_of(dy2) == yw, "structure2 from store(): run bit-exact");
+      wko_free(L2);
+    }
+  }
+  wko_free(L1);
+}
+
 int main(int argc, char **argv) {
   const int n = argc > 1 ? std::atoi(argv[1]) : 16;
   const int nodes = (n + 1) * (n + 1) * (n + 1);
@@ -201,6 +460,8 @@ int main(int argc, char **argv) {
     std::printf("FP32 compensated: normwise error vs FP64 on the same inputs %.3e\n", std::sqrt(num / den));
     expect(std::sqrt(num / den) <= 1e-10, "FP32 compensated error");
   }
+
+  drivers_verbatim(rowptr, colind, vals, x, y, M, N, nz);
 
   // errors surface as exceptions
   try {

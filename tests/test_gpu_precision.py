@@ -51,6 +51,13 @@ FP32_SHAPES = {
          dict(WIN, k_tile=8, unroll=8, waves_per_block=8, stage_lds=4),
          dict(k_tile=32, stage_lds=0, unroll=8, waves_per_block=2)],
 }
+# values held in registers across 32-byte k-tiles (8 FP32 / 4 FP64 right-hand
+# sides per pass), one to four waves per block
+for _k in (4, 8, 16, 32):
+    FP32_SHAPES[_k] += [dict(WIN, k_tile=8, unroll=8, waves_per_block=w, stage_lds=st)
+                        for w in (1, 4) for st in (6, 7)]
+FP64_SHAPES = {k: [dict(WIN, k_tile=min(k, 8), unroll=8, waves_per_block=w, stage_lds=st, xcd_remap=r)
+                   for w, r in ((1, 1), (2, 0), (4, 1)) for st in (6, 7)] for k in KS}
 
 
 def say(*a):
@@ -132,8 +139,9 @@ def test_fp64_multi_rhs_bitexact(hex32, variant, k):
     dflt = hex32[("default", variant)]
     tuned, _ = e64.tune(xd, y, reps=3)
     tuned_x, _ = e64.tune(xrd, y, op=1, reps=3)
-    for name, cfg, cfg_x in (("default", dflt[("f64", 0, k)], dflt[("f64", 1, k)]),
-                             ("tuned", tuned, tuned_x)):
+    shapes = [("default", dflt[("f64", 0, k)], dflt[("f64", 1, k)]), ("tuned", tuned, tuned_x)]
+    shapes += [(f"shape{i}", c, c) for i, c in enumerate(FP64_SHAPES[k])]
+    for name, cfg, cfg_x in shapes:
         yr, _ = run_both(e64, xd, xrd, m.nrows, k, cfg)
         _, yx = run_both(e64, xd, xrd, m.nrows, k, cfg_x)
         assert np.array_equal(yr, ref), (name, cfg)

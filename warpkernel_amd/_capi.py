@@ -75,7 +75,10 @@ SIGNATURES = {
     "wk_mm_write": (C.c_int, [C.c_char_p, I32, I32, I64, P, P, P]),
     "wk_layout_save": (C.c_int, [P, C.c_char_p, C.c_char_p]),
     "wk_layout_load": (C.c_int, [C.c_char_p, C.POINTER(P), C.c_char_p, I64]),
+    "wk_layout_from_arrays": (C.c_int, [I32, I32, I32, I32, I32, I32, I32, I32, I64, I64, P, P, P, P, P,
+                                        P, P, PP]),
     "wk_layout_destroy": (None, [P]),
+    "wk_scatter": (C.c_int, [P, P, P, I64, I64, I32, P]),
     "wk_engine_create": (C.c_int, [P, P, P, P, I32, U32, P, PP]),
     "wk_engine_destroy": (None, [P]),
     "wk_engine_get_info": (C.c_int, [P, C.POINTER(LayoutInfo)]),

@@ -3,7 +3,9 @@
 // (engine<V,I>, warpkernel.hpp:773-959).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "wk_internal.hpp"
@@ -247,6 +249,61 @@ int wk_layout_reorder_values_rowsort(const wk_layout *L, const double *vals_host
       fail(WK_ERR_STATE, "call wk_layout_reorder_columns_rowsort first");
     std::memset(pvals_host, 0, sizeof(double) * (size_t)L->allocate_nz);
     for (int64_t i = 0; i < L->nz; i++) pvals_host[L->reorder_rows[i]] = vals_host[L->reorder_A_rows[i]];
+  });
+}
+
+int wk_layout_from_arrays(int32_t variant, int32_t group_width, int32_t threshold, int32_t max_tpr,
+                          int32_t nrows, int32_t ncols, int32_t nznrows, int32_t nwarps, int64_t nz,
+                          int64_t allocate_nz, const int32_t *rowptr, const int32_t *row_map,
+                          const uint32_t *reorder_rows, const int64_t *warp_offsets,
+                          const uint32_t *max_nz_rows, const uint32_t *threads_per_row,
+                          const uint32_t *row_offset_warp, wk_layout **out) {
+  return guarded([&] {
+    if (!out) fail(WK_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (nrows < 0 || nwarps < 0 || nz < 0) fail(WK_ERR_ARG, "negative size");
+    if (!rowptr || (nrows && !row_map) || (nz && !reorder_rows) || !warp_offsets ||
+        (nwarps && !max_nz_rows) || (variant == 2 && nwarps && (!threads_per_row || !row_offset_warp)))
+      fail(WK_ERR_ARG, "NULL array");
+    auto L = std::make_unique<wk_layout>();
+    L->variant = variant;
+    L->W = group_width;
+    L->threshold = threshold;
+    L->max_tpr = max_tpr;
+    L->nrows = nrows;
+    L->ncols = ncols;
+    L->nznrows = nznrows;
+    L->nwarps = nwarps;
+    L->nz = nz;
+    L->allocate_nz = allocate_nz;
+    L->rowptr.assign(rowptr, rowptr + nrows + 1);
+    L->row_map.assign(row_map, row_map + nrows);
+    L->row_map_inv.assign((size_t)nrows, -1);
+    for (int32_t i = 0; i < nrows; i++) {
+      const int32_t r = row_map[i];
+      if (r < 0 || r >= nrows || L->row_map_inv[r] != -1) fail(WK_ERR_CSR, "layout arrays: row_map is not a permutation");
+      L->row_map_inv[r] = i;
+    }
+    L->reorder_rows.assign(reorder_rows, reorder_rows + nz);
+    L->warp_offsets.assign(warp_offsets, warp_offsets + nwarps + 1);
+    L->max_nz_rows.assign(max_nz_rows, max_nz_rows + nwarps);
+    if (variant == 2) {
+      L->threads_per_row.assign(threads_per_row, threads_per_row + nwarps);
+      L->row_offset_warp.assign(row_offset_warp, row_offset_warp + nwarps + 1);
+    }
+    check_layout(*L, "layout arrays");
+    if (ncols < 0) fail(WK_ERR_CSR, "layout arrays: ncols");
+    if (variant == 2) {
+      // over the rows with entries, the shortest starting from nrows (hpp:607)
+      L->min_nz = (uint32_t)nrows;
+      L->max_nz = 0;
+      for (int32_t r = 0; r < nrows; r++) {
+        const uint32_t len = (uint32_t)(rowptr[r + 1] - rowptr[r]);
+        if (len) L->min_nz = std::min(L->min_nz, len);
+        L->max_nz = std::max(L->max_nz, len);
+      }
+    }
+    *out = L.release();
   });
 }
 
@@ -506,6 +563,17 @@ int wk_run_assembly(wk_engine *E, const wk_launch *cfg, double *y_dev, void *str
     if (!y_dev && E->nrows) fail(WK_ERR_ARG, "NULL vector");
     launch_rowsum(E->fwd, E->dtype, resolve(E, WK_OP_RUN, 1, cfg), y_dev,
                   static_cast<hipStream_t>(stream));
+  });
+}
+
+int wk_scatter(const void *src_dev, const uint32_t *index_dev, void *dst_dev, int64_t n,
+               int64_t dst_count, int32_t elem_bytes, void *stream) {
+  return guarded([&] {
+    if (n < 0 || dst_count < 0) fail(WK_ERR_ARG, "negative size");
+    if (elem_bytes != 4 && elem_bytes != 8) fail(WK_ERR_ARG, "elem_bytes must be 4 or 8");
+    if (n && (!src_dev || !index_dev || (dst_count && !dst_dev))) fail(WK_ERR_ARG, "NULL array");
+    launch_scatter(src_dev, index_dev, dst_dev, n, dst_count, elem_bytes,
+                   static_cast<hipStream_t>(stream));
   });
 }
 

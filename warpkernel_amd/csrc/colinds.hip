@@ -20,7 +20,6 @@ namespace wk {
 namespace {
 
 constexpr int kFlagThreads = 256;
-
 // flags[b] = 1 if block b saw an index outside [0, bound)
 __global__ __launch_bounds__(kFlagThreads) void range_check_kernel(const int32_t *__restrict__ a,
                                                                    int64_t n, int32_t bound,
@@ -75,6 +74,7 @@ __global__ __launch_bounds__(kFlagThreads) void window_match_kernel(
 // synchronised.
 template <class F>
 bool any_flag(int32_t *flags, int32_t nb, hipStream_t s, const char *what, F &&launch) {
+  if (nb > kColFlags) fail(WK_ERR_LIMIT, "more check blocks than flags");
   launch();
   hipError_t e = hipGetLastError();
   int32_t h[kColFlags];

@@ -236,9 +236,11 @@ void layout_save(const wk_layout &L, const char *path, const char *driver_name) 
   if (!w.f) fail(WK_ERR_ARG, std::string("write failed: ") + path);
 }
 
+}  // namespace
+
 // Every invariant the kernels and the host helpers index by.
-void check_layout(const wk_layout &L) {
-  auto bad = [](const std::string &m) { fail(WK_ERR_CSR, "layout file: " + m); };
+void check_layout(const wk_layout &L, const char *what) {
+  auto bad = [what](const std::string &m) { fail(WK_ERR_CSR, std::string(what) + ": " + m); };
   if (L.variant != 1 && L.variant != 2) bad("variant");
   if (L.W != 32 && L.W != 64) bad("group width");
   if (L.nrows < 0 || L.ncols < 0 || L.nznrows < 0 || L.nznrows > L.nrows || L.nwarps < 0) bad("sizes");
@@ -281,6 +283,8 @@ void check_layout(const wk_layout &L) {
       if (s < 0 || (int64_t)s >= L.nz) bad("reorder_A_rows outside the matrix");
   }
 }
+
+namespace {
 
 void layout_load(const char *path, wk_layout &L, std::string &driver_name) {
   Reader r;
@@ -325,7 +329,7 @@ void layout_load(const char *path, wk_layout &L, std::string &driver_name) {
   uint64_t h = 0;
   if (!r.f.read(reinterpret_cast<char *>(&h), sizeof h) || h != r.sum.h)
     fail(WK_ERR_CSR, "layout file checksum mismatch");
-  check_layout(L);
+  check_layout(L, "layout file");
 }
 
 }  // namespace

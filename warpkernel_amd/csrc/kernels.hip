@@ -392,6 +392,109 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_winpk_kernel(SpmvArgs a, Win
     a.y[dst * a.ldy] = res;
 }
 
+// Several right-hand sides with the group's values held in registers: each
+// wave loads its lane's values and packed window positions once (at most
+// 12 * NB steps), then makes one pass per k-tile of KT right-hand sides: fill
+// the x window with the tile's KT values per column (KT * sizeof(X) = 32
+// bytes per entry), sum every slot from registers and LDS, store the tile.
+// The values stream from HBM once whatever k, the window is a quarter (FP64)
+// of the 8-wide one, and the only LDS traffic is the x reads and the fill.
+// Per lane and right-hand side the order is the other kernels' (every slot by
+// fma onto -0, pads included), so results are bit-identical.  A pass reuses
+// the window of the previous one: the wave's LDS operations complete in
+// order, so the fill's writes follow the previous pass's reads.
+template <class P, bool SPLIT, int OUT, int NB>
+__global__ __launch_bounds__(kMaxThreads) void spmv_winres_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
+                                                                  int32_t ntiles, int32_t remap) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  constexpr int W = 64;
+  constexpr int KT = 32 / (int)sizeof(X);  // right-hand sides per pass
+  constexpr int NS = 12 * NB;              // steps held in registers
+  constexpr int kAhead = 4;
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t wpb = blockDim.x >> 6;
+  const int32_t bid = blockIdx.x;
+  if (bid >= nwork) {
+    const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
+    if (pos >= a.nrows) return;
+    const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
+    for (int32_t t = 0; t < ntiles * KT; t++)
+      a.y[dst * a.ldy + t] = OUT == OUT_RESID ? a.b[dst * a.ldb + t] : 0.0;
+    return;
+  }
+  const int32_t wib = threadIdx.x >> 6;
+  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(bid, nwork, remap) * wpb + wib);
+  if (wave >= a.ngroups) return;
+  const GroupMeta m = load_meta<W, SPLIT>(a, wave);  // m.len <= NS (checked by the launcher)
+  const V *vp = static_cast<const V *>(a.vals) + m.base + lane;
+  const u32x4 *pk = reinterpret_cast<const u32x4 *>(w.pk) + (int64_t)w.pk_off[wave] * 64 + lane;
+  // the lane's positions and values, all loads issued before the first use
+  u32x4 pw[NB];
+  V v[NS];
+#pragma unroll
+  for (int q = 0; q < NB; q++)
+    if ((uint32_t)(12 * q) < m.len) pw[q] = pk[q * 64];
+#pragma unroll
+  for (int i = 0; i < NS; i++)
+    if ((uint32_t)i < m.len) v[i] = __builtin_nontemporal_load(vp + (int64_t)i * W);
+  char *xs = lds + wib * w.wave_bytes;
+  const uint32_t w0 = w.off[wave], nwin = w.off[wave + 1] - w0;
+  const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
+  const bool writer = ((uint32_t)lane & (m.tpr - 1)) == 0 && rowid < (uint32_t)a.nznrows;
+  const int64_t dst = !writer ? 0 : OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
+  for (int32_t t = 0; t < ntiles; t++) {
+    // the positions as if new each pass: otherwise every step's window
+    // address is hoisted out of this loop and held in registers
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      uint32_t c0 = pw[q].x, c1 = pw[q].y, c2 = pw[q].z, c3 = pw[q].w;
+      asm volatile("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(c3));
+      pw[q] = u32x4{c0, c1, c2, c3};
+    }
+    WinFill<X, KT, 64, 32> f;
+    f.cols(w.cols + w0, nwin, lane);
+    f.gather(xs, w.cols + w0, nwin, static_cast<const X *>(a.x) + t * KT, a.ldx, lane);
+    Acc acc[KT];
+#pragma unroll
+    for (int j = 0; j < KT; j++) P::init(acc[j]);  // every slot by fma onto -0
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      if ((uint32_t)i < m.len) {  // wave-uniform
+        X xv[KT];
+        WinEntry<X, KT>::read(xs, WinBatchPk<P, NB>::pos(pw[i / 12], i % 12), xv);
+#pragma unroll
+        for (int j = 0; j < KT; j++) P::step(acc[j], v[i], xv[j]);
+      }
+      // at most kAhead steps of window reads scheduled ahead of their FMAs:
+      // the values already take 2 * NS registers (FP64)
+      if (i % kAhead == kAhead - 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (SPLIT) {
+      for (uint32_t o = 1; o < m.tpr; o <<= 1) {
+#pragma unroll
+        for (int j = 0; j < KT; j++) {
+          Acc other = P::shfl_down(acc[j], (int)o, W);
+          P::add(acc[j], other);
+        }
+      }
+    }
+    if (writer) {
+      const int64_t c0 = (int64_t)t * KT;
+#pragma unroll
+      for (int j = 0; j < KT; j++) {
+        const double res = P::result(acc[j]);
+        if constexpr (OUT == OUT_RESID)
+          a.y[dst * a.ldy + c0 + j] = a.b[dst * a.ldb + c0 + j] - res;
+        else
+          a.y[dst * a.ldy + c0 + j] = res;
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------ small kernels
 // Streaming read of a buffer: 16-byte nontemporal loads, 8 per thread issued
 // before use.  The XOR of the data only decides whether one word is stored,
@@ -426,6 +529,18 @@ __global__ void scatter_kernel(const T *__restrict__ src, const uint32_t *__rest
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     dst[slot[i]] = src[i];
+}
+
+// thrust::scatter of the reference's _GPU helpers (warpkernel.hpp:459-502),
+// bounded: indices at or beyond dst_count are skipped.
+template <class T>
+__global__ void scatter_bounded_kernel(const T *__restrict__ src, const uint32_t *__restrict__ index,
+                                       T *__restrict__ dst, int64_t n, int64_t dst_count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t j = index[i];
+    if ((int64_t)j < dst_count) dst[j] = src[i];
+  }
 }
 
 // dvals[t] = sum_j dy[row_t][j] * x[col_t][j], CSR order.  With one
@@ -869,6 +984,38 @@ WinPkFn pick_winpk(int prec, bool split, int out, int nb) {
   return nb == 1 ? pick_winpk_split<PF32D, 1>(split, out) : pick_winpk_split<PF32D, 2>(split, out);
 }
 
+using WinResFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
+                          int32_t, hipStream_t);
+
+template <class P, bool SPLIT, int OUT, int NB>
+void launch_winres(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
+                   int32_t nwork, int32_t ntiles, int32_t remap, hipStream_t s) {
+  hipLaunchKernelGGL((spmv_winres_kernel<P, SPLIT, OUT, NB>), grid, block, lds, s, a, w, nwork,
+                     ntiles, remap);
+}
+
+template <class P, bool SPLIT, int NB>
+WinResFn pick_winres_out(int out) {
+  if (out == OUT_RUN) return &launch_winres<P, SPLIT, OUT_RUN, NB>;
+  if (out == OUT_RUNX) return &launch_winres<P, SPLIT, OUT_RUNX, NB>;
+  if constexpr (std::is_same_v<P, PF64>) return &launch_winres<P, SPLIT, OUT_RESID, NB>;
+  return nullptr;
+}
+
+template <class P>
+WinResFn pick_winres_p(bool split, int out, int nb) {
+  if (nb == 3) return split ? pick_winres_out<P, true, 3>(out) : pick_winres_out<P, false, 3>(out);
+  if (nb == 7) return split ? pick_winres_out<P, true, 7>(out) : pick_winres_out<P, false, 7>(out);
+  return nullptr;
+}
+
+// nb: 12-step blocks held in registers (3 or 7: groups of up to 36 or 84 steps)
+WinResFn pick_winres(int prec, bool split, int out, int nb) {
+  if (prec == 0) return pick_winres_p<PF64>(split, out, nb);
+  if (prec == 1) return pick_winres_p<PF32C>(split, out, nb);
+  return pick_winres_p<PF32D>(split, out, nb);
+}
+
 size_t staged_lds_per_wave(int prec, int u) {
   return (size_t)u * 64 * ((prec == 0 ? sizeof(double) : sizeof(float)) + sizeof(int32_t));
 }
@@ -885,7 +1032,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
                  int64_t ldx, double *y, int64_t ldy, const double *b, int64_t ldb, int32_t k,
                  hipStream_t stream) {
   if (k < 1) fail(WK_ERR_ARG, "k must be >= 1");
-  if (cfg.stage_lds < 0 || cfg.stage_lds > 5) fail(WK_ERR_ARG, "stage_lds must be 0 to 5");
+  if (cfg.stage_lds < 0 || cfg.stage_lds > 6) fail(WK_ERR_ARG, "stage_lds must be 0 to 6");
   const int wpb = cfg.waves_per_block;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) fail(WK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
   if (cfg.k_tile < 1 || cfg.k_tile > 32 || (cfg.k_tile & (cfg.k_tile - 1)))
@@ -918,9 +1065,40 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     lds = std::min<size_t>((size_t)lim.per_cu / (size_t)cfg.blocks_per_cu, (size_t)lim.per_block);
   }
 
-  // k = sum of power-of-two tiles: as many k_tile tiles as fit, then the rest
   int32_t j0 = 0;
-  int kt = std::min(cfg.k_tile, floor_pow2(k));
+  // stage 6: the group's values in registers, one pass per 32-byte k-tile
+  // (4 FP64 or 8 FP32 right-hand sides), all passes of a group in one wave;
+  // the rest of k (fewer than one tile) continues below
+  if (cfg.stage_lds == 6 && D.win_max > 0 && D.win_pk && D.W == 64 && rpl == 1 && cfg.nontemporal) {
+    const int ktr = 32 / (int)xsz;
+    const int nb = D.pk_longest <= 36 ? 3 : D.pk_longest <= 84 ? 7 : 0;
+    const int32_t ntiles = k / ktr;
+    const size_t win_bytes = ((size_t)D.win_max * 32 + 15) / 16 * 16;
+    WinResFn rf = nb && ntiles ? pick_winres(prec, split, out, nb) : nullptr;
+    if (rf && win_bytes * (size_t)wpb <= (size_t)lds_limits().per_block) {
+      WinArgs w{};
+      w.off = D.win_off;
+      w.cols = (out == OUT_RUNX && D.win_cols_kr) ? D.win_cols_kr : D.win_cols;
+      w.wave_bytes = (int32_t)win_bytes;
+      w.win_bytes = (int32_t)win_bytes;
+      w.pk = D.win_pk;
+      w.pk_off = D.win_pk_off;
+      const int64_t nwork = ((int64_t)D.ngroups + wpb - 1) / wpb;
+      const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+      if (nwork + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+      a.x = x;
+      a.y = y;
+      a.b = b;
+      a.k = ntiles * ktr;
+      if (nwork + nf > 0)
+        rf(a, w, dim3((uint32_t)(nwork + nf)), dim3(wpb * 64), std::max(win_bytes * (size_t)wpb, lds),
+           (int32_t)nwork, ntiles, cfg.xcd_remap ? 1 : 0, stream);
+      hip_check(hipGetLastError(), "spmv launch");
+      j0 = ntiles * ktr;
+    }
+  }
+  // k = sum of power-of-two tiles: as many k_tile tiles as fit, then the rest
+  int kt = std::min(cfg.k_tile, floor_pow2(k - j0 > 0 ? k - j0 : 1));
   if (D.W == 32) kt = 1;
   while (j0 < k) {
     const int32_t rem = k - j0;
@@ -936,7 +1114,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const bool windowed = cfg.stage_lds >= 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
                           cfg.nontemporal &&
                           (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
-    if (windowed && cfg.stage_lds == 5 && kt == 1 && D.win_pk) {
+    if (windowed && cfg.stage_lds >= 5 && kt == 1 && D.win_pk) {  // 5, and the rest of 6
       // packed positions: per wave the window, then a 12-step slice of values
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
       const size_t win_bytes = ((size_t)D.win_max * xsz + 15) / 16 * 16;
@@ -975,10 +1153,10 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       // multiple), then a U-step slice of values and 16-bit positions, one
       // group per wave.  Wider k-tiles: the block's waves share one group's
       // window, each with its own slice.
-      const int u = kt == 1 ? cfg.unroll : (cfg.stage_lds == 4 && cfg.unroll == 16 ? 16 : 8);
+      const int u = kt == 1 ? cfg.unroll : ((cfg.stage_lds == 4 || cfg.stage_lds >= 6) && cfg.unroll == 16 ? 16 : 8);
       // waves per window: waves_per_block, at most one per 16-byte chunk of
       // an entry, or (FP32, values in registers) one per right-hand side
-      const bool regs = cfg.stage_lds == 4;  // no slices
+      const bool regs = cfg.stage_lds == 4 || cfg.stage_lds >= 6;  // no slices
       const int chunks = regs && xsz == 4 ? kt : std::max(1, kt * (int)xsz / 16);
       const int wpw = kt == 1 ? 1 : std::min(wpb, chunks);
       const int bw = wpw > 1 ? wpw : wpb;  // waves per block
@@ -1094,7 +1272,10 @@ void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream) {
   if (D.W != 64 || D.win_max <= 0 || D.win_max > kPackedWindow || !D.win_idx || D.ngroups <= 0)
     return;
   std::vector<uint32_t> off((size_t)D.ngroups + 1, 0);
-  for (int32_t g = 0; g < D.ngroups; g++) off[g + 1] = off[g] + (D.h_max_nz_rows[g] + 11) / 12;
+  for (int32_t g = 0; g < D.ngroups; g++) {
+    off[g + 1] = off[g] + (D.h_max_nz_rows[g] + 11) / 12;
+    D.pk_longest = std::max(D.pk_longest, D.h_max_nz_rows[g]);
+  }
   D.win_pk_off = dev_upload(E, off.data(), (int64_t)off.size(), stream);
   D.win_pk = dev_alloc<uint32_t>(E, (int64_t)off.back() * 64 * 4);
   constexpr int wpb = 4;
@@ -1130,6 +1311,21 @@ void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const
     hipLaunchKernelGGL(scatter_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream,
                        static_cast<const float *>(vals_csr), D.slot_of_nz,
                        static_cast<float *>(D.vals), nnz);
+  hip_check(hipGetLastError(), "scatter launch");
+}
+
+void launch_scatter(const void *src, const uint32_t *index, void *dst, int64_t n, int64_t dst_count,
+                    int32_t elem_bytes, hipStream_t stream) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL(scatter_bounded_kernel<uint64_t>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       static_cast<const uint64_t *>(src), index, static_cast<uint64_t *>(dst), n,
+                       dst_count);
+  else
+    hipLaunchKernelGGL(scatter_bounded_kernel<uint32_t>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       static_cast<const uint32_t *>(src), index, static_cast<uint32_t *>(dst), n,
+                       dst_count);
   hip_check(hipGetLastError(), "scatter launch");
 }
 

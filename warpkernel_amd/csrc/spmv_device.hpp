@@ -402,10 +402,13 @@ struct WinEntry {
 // columns.  S is 64 times the number of waves that fill one window together.
 // C keeps one round's gathered values within about 96 registers, and one
 // round covers 768 entries (hex elasticity windows hold 646 on average).
-template <class X, int KT, int S = 64>
+// REGS bounds the registers one round of gathered values takes (kernels that
+// hold other data in registers pass less).
+template <class X, int KT, int S = 64, int REGS = 96>
 struct WinFill {
   static constexpr int kRegs = KT * (int)sizeof(X) / 4;  // registers per entry
-  static constexpr int kByRegs = 96 / kRegs > 12 ? 12 : (96 / kRegs < 4 ? 4 : 96 / kRegs);
+  static constexpr int kLow = REGS >= 4 * kRegs ? 4 : (REGS / kRegs < 1 ? 1 : REGS / kRegs);
+  static constexpr int kByRegs = REGS / kRegs > 12 ? 12 : (REGS / kRegs < kLow ? kLow : REGS / kRegs);
   static constexpr int kByCover = 768 / S > 1 ? 768 / S : 1;
   static constexpr int C = kByRegs < kByCover ? kByRegs : kByCover;
   int32_t c[C];

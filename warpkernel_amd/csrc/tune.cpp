@@ -92,6 +92,14 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
             c.stage_lds = 0;
           }
         }
+        // the group's values held in registers across 32-byte k-tiles (the
+        // launcher falls back where groups are longer than 84 steps)
+        if (D.win_pk && k >= (int)(32 / vsz) && wpb <= 4) {
+          c.stage_lds = 6;
+          c.k_tile = std::min(32, pow2_at_most(k));
+          out.push_back(c);
+          c.stage_lds = 0;
+        }
       }
     }
   return out;
@@ -171,6 +179,15 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         c.waves_per_block = 2;
       } else {
         c.stage_lds = 3;
+        c.k_tile = 8;
+        c.waves_per_block = 1;
+      }
+      if (D.win_pk && D.pk_longest <= 84 && mean_len >= 40.0 &&
+          (k >= 32 || (k >= 16 && D.ngroups < 4096))) {
+        // the group's values in registers across 4-wide passes (stage 6):
+        // elasticity 32^3 k=16 58.3 against 61.1 us, k=32 99.9 against 111.6;
+        // 64^3 k=32 803 against 819, but k=16 446 against 423
+        c.stage_lds = 6;
         c.k_tile = 8;
         c.waves_per_block = 1;
       }

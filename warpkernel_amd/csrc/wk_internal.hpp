@@ -64,6 +64,9 @@ void reorder_columns_rowsort(const wk_layout &L, const int32_t *rowptr, const in
 void transpose_csr(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *colind,
                    std::vector<int32_t> &colptr, std::vector<int32_t> &rowind,
                    std::vector<int64_t> &perm);
+// Fails with WK_ERR_CSR ("<what>: ...") unless the layout satisfies every
+// invariant the kernels and the host helpers index by.
+void check_layout(const wk_layout &L, const char *what);
 // Run fn(begin, end) over [0, n) on up to 16 host threads.
 void parallel_for(int64_t n, int64_t min_chunk, const std::function<void(int64_t, int64_t)> &fn);
 
@@ -102,6 +105,7 @@ struct DevLayout {
   // x 16 bytes, lane l's block holding its 12 positions of those 12 steps
   uint32_t *win_pk = nullptr;      // [win_pk_off[ngroups] * 64 * 4]
   uint32_t *win_pk_off = nullptr;  // [ngroups+1] first 12-step block of each group
+  uint32_t pk_longest = 0;         // the longest group (steps), with win_pk
 };
 
 // Window positions fit the packed form up to this many entries per window.
@@ -170,6 +174,10 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
 void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
                            hipStream_t stream);
 void launch_stream_read(const void *p, int64_t bytes, uint32_t *sink, hipStream_t stream);
+// dst[index[i]] = src[i] for i < n with index[i] < dst_count (others skipped);
+// elements of elem_bytes (4 or 8) bytes.
+void launch_scatter(const void *src, const uint32_t *index, void *dst, int64_t n, int64_t dst_count,
+                    int32_t elem_bytes, hipStream_t stream);
 // Builds D.win_pk / D.win_pk_off from D.win_idx when the windows fit the
 // packed form (width 64, win_max <= kPackedWindow); otherwise leaves them null.
 void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream);
