@@ -231,6 +231,17 @@ int wk_engine_update_values(wk_engine *engine, const void *vals_dev, void *strea
 int wk_engine_set_colinds(wk_engine *engine, const int32_t *pcols, int32_t where, void *stream);
 int wk_engine_get_colinds(const wk_engine *engine, int32_t *pcols_host);
 
+/* wk_engine_set_colinds with device columns (where = 1), allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate (loops
+ * that swap an engine's columns while serving, harnesses that forbid
+ * hipMalloc).  The column checks write their flags into a fixed device array
+ * of the library, 1024 int32 (4 KB), one per device, held until the process
+ * ends; the checks never need more flags, whatever allocate_nz, so no input
+ * exceeds it.  Every call on one device shares that array: calls that overlap
+ * on one device race.  Same results, errors and window handling as
+ * wk_engine_set_colinds; synchronises `stream`. */
+int wk_engine_set_colinds_noalloc(wk_engine *engine, const int32_t *pcols_dev, void *stream);
+
 /* Entries over all x windows the engine's forward launches use: 0 when it has
  * none, or while its columns differ from the scanned ones (see
  * wk_engine_set_colinds).  No reference counterpart. */

@@ -85,6 +85,7 @@ SIGNATURES = {
     "wk_engine_update_values": (C.c_int, [P, P, P]),
     "wk_engine_set_colinds": (C.c_int, [P, P, I32, P]),
     "wk_engine_x_windows": (C.c_int, [P, C.POINTER(I64)]),
+    "wk_engine_set_colinds_noalloc": (C.c_int, [P, P, P]),
     "wk_engine_get_colinds": (C.c_int, [P, P]),
     "wk_engine_create_device": (C.c_int, [I32, I32, I32, I32, I32, I64, P, P, P, I32, C.c_uint32, P,
                                           C.POINTER(P)]),

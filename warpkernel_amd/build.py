@@ -29,8 +29,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread"]
 
-LIB_SOURCES = ["kernels.hip", "batch.hip", "dscan.hip", "assembly.hip", "colinds.hip", "capi.cpp", "scan.cpp",
-               "tune.cpp", "io.cpp"]
+LIB_SOURCES = ["kernels.hip", "batch.hip", "dscan.hip", "assembly.hip", "colinds.hip", "noalloc.hip", "capi.cpp",
+               "scan.cpp", "tune.cpp", "io.cpp"]
 
 
 def _run(cmd: list[str]) -> None:
