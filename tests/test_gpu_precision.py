@@ -53,11 +53,10 @@ FP32_SHAPES = {
 }
 # values held in registers across 32-byte k-tiles (8 FP32 / 4 FP64 right-hand
 # sides per pass), one to four waves per block
-for _k in (4, 8, 16, 32):
-    FP32_SHAPES[_k] += [dict(WIN, k_tile=8, unroll=8, waves_per_block=w, stage_lds=st)
-                        for w in (1, 4) for st in (6, 7)]
-FP64_SHAPES = {k: [dict(WIN, k_tile=min(k, 8), unroll=8, waves_per_block=w, stage_lds=st, xcd_remap=r)
-                   for w, r in ((1, 1), (2, 0), (4, 1)) for st in (6, 7)] for k in KS}
+for _k in (8, 16, 32):
+    FP32_SHAPES[_k] += [dict(WIN, k_tile=8, unroll=8, waves_per_block=w, stage_lds=6) for w in (1, 4)]
+FP64_SHAPES = {k: [dict(WIN, k_tile=min(k, 8), unroll=8, waves_per_block=w, stage_lds=6, xcd_remap=r)
+                   for w, r in ((1, 1), (2, 0), (4, 1))] for k in KS}
 
 
 def say(*a):
