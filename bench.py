@@ -554,6 +554,8 @@ def run_case(args, torch, dev):
         eng = engine(s, m.vals, m.colind)
         x = dev(fe.uniform_x(m.ncols, seed=2300))
         y = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+        if args.launch:
+            eng.set_launch(json.loads(args.launch))
         fns = [lambda: eng.launch(x, y)]
         info["launch"] = eng.get_launch().as_dict()
     elif name.startswith("batch64"):
@@ -685,6 +687,8 @@ def main():
     ap.add_argument("--subset", choices=["all", "windowed", "direct"], default="all",
                     help="batch64 case: all matrices, or those with (without) k > 1 windows")
     ap.add_argument("--batch-stage", type=int, default=5, help="batch64 case: stage_lds")
+    ap.add_argument("--launch", default=None,
+                    help="tet96 / tet96_perm cases: launch fields to override (JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--write-traffic", nargs=4, metavar=("WORKLOAD", "FETCH_DIR", "WRITE_DIR", "CASE_JSON"))

@@ -290,7 +290,13 @@ typedef struct wk_launch {
                               and 16 with stage_lds 4 at k_tiles 4 and 8) */
   int32_t k_tile;          /* right-hand sides per pass: 1, 2, 4, 8, 16 or 32 */
   int32_t xcd_remap;       /* 1: contiguous group ranges per XCD */
-  int32_t nontemporal;     /* 1: streaming loads for values and indices */
+  int32_t nontemporal;     /* 1: streaming (nontemporal) loads for the values;
+                              2: for the column indices too (direct loads at
+                              k_tile 1, unroll 8, rows_per_lane 1 or 2;
+                              elsewhere as 1), which leaves more of L2 to
+                              scattered x gathers; 0: plain loads (direct
+                              loads at k_tile 1, unroll 8, rows_per_lane 1
+                              only: other shapes are refused) */
   int32_t accumulate;      /* WK_ACC_* for FP32 engines */
   int32_t rows_per_lane;   /* 1, 2 or 4 groups per wave (k_tile 1, group width 64) */
   int32_t blocks_per_cu;   /* 0: no cap; else resident blocks per CU are capped

@@ -137,7 +137,10 @@ def test_dropin_driver_flow_wpk1():
 
 @pytest.mark.parametrize("cfg", [
     dict(waves_per_block=1), dict(waves_per_block=8), dict(unroll=4), dict(unroll=16),
-    dict(xcd_remap=0), dict(nontemporal=0), dict(rows_per_lane=2), dict(rows_per_lane=2, unroll=4),
+    dict(xcd_remap=0), dict(nontemporal=0), dict(nontemporal=2), dict(nontemporal=2, waves_per_block=4),
+    dict(nontemporal=2, rows_per_lane=2), dict(nontemporal=2, unroll=16),
+    dict(nontemporal=2, rows_per_lane=2, unroll=4),
+    dict(rows_per_lane=2), dict(rows_per_lane=2, unroll=4),
     dict(rows_per_lane=4, unroll=4), dict(rows_per_lane=4), dict(blocks_per_cu=3),
     dict(rows_per_lane=2, waves_per_block=1, blocks_per_cu=6),
     dict(stage_lds=1), dict(stage_lds=1, unroll=16), dict(stage_lds=1, waves_per_block=1, unroll=16),

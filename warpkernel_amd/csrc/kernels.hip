@@ -68,7 +68,7 @@ __device__ __forceinline__ GroupMeta load_meta(const SpmvArgs &a, int32_t g) {
 // wave owns R consecutive groups ("rows per lane" = R); with W = 32 it owns
 // two groups, one per half-wave.  Fill blocks write the rows without nonzeros
 // (sorted positions nznrows..nrows-1).
-template <class P, int W, bool SPLIT, int OUT, int KT, int U, bool NT, int R>
+template <class P, int W, bool SPLIT, int OUT, int KT, int U, int NT, int R>
 __global__ __launch_bounds__(kMaxThreads) void spmv_kernel(SpmvArgs a, int32_t nwork,
                                                            int32_t ktiles, int32_t remap) {
   using V = typename P::V;
@@ -734,7 +734,7 @@ __global__ __launch_bounds__(kDvThreads) void backward_kernel(SpmvArgs a, int32_
 using LaunchFn = void (*)(const SpmvArgs &, dim3, dim3, size_t, int32_t, int32_t, int32_t,
                           hipStream_t);
 
-template <class P, int W, bool SPLIT, int OUT, int KT, int U, bool NT, int R>
+template <class P, int W, bool SPLIT, int OUT, int KT, int U, int NT, int R>
 void launch_one(const SpmvArgs &a, dim3 grid, dim3 block, size_t lds, int32_t nwork,
                 int32_t ktiles, int32_t remap, hipStream_t s) {
   hipLaunchKernelGGL((spmv_kernel<P, W, SPLIT, OUT, KT, U, NT, R>), grid, block, lds, s, a,
@@ -745,13 +745,17 @@ void launch_one(const SpmvArgs &a, dim3 grid, dim3 block, size_t lds, int32_t nw
 // rows-per-lane and load-policy choices; wider k-tiles carry more registers
 // per slot and keep one shape each.
 template <class P, int W, bool SPLIT, int OUT>
-LaunchFn pick_kt(int kt, int u, bool nt, int r) {
+LaunchFn pick_kt(int kt, int u, int nt, int r) {
   if constexpr (W == 32) {
     // the reference's own layout width: parity path, one right-hand side per pass
     return kt == 1 && r == 1 ? &launch_one<P, 32, SPLIT, OUT, 1, 8, true, 1> : nullptr;
   } else {
     if (kt == 1) {
-      if (!nt) return r == 1 && u == 8 ? &launch_one<P, W, SPLIT, OUT, 1, 8, false, 1> : nullptr;
+      if (nt == 0) return r == 1 && u == 8 ? &launch_one<P, W, SPLIT, OUT, 1, 8, 0, 1> : nullptr;
+      // nontemporal index loads at unroll 8, one or two groups per lane;
+      // elsewhere nontemporal 2 loads as 1
+      if (nt == 2 && u == 8 && r == 1) return &launch_one<P, W, SPLIT, OUT, 1, 8, 2, 1>;
+      if (nt == 2 && u == 8 && r == 2) return &launch_one<P, W, SPLIT, OUT, 1, 8, 2, 2>;
       if (r == 1) {
         if (u == 4) return &launch_one<P, W, SPLIT, OUT, 1, 4, true, 1>;
         if (u == 8) return &launch_one<P, W, SPLIT, OUT, 1, 8, true, 1>;
@@ -778,7 +782,7 @@ LaunchFn pick_kt(int kt, int u, bool nt, int r) {
 }
 
 template <class P, int W, bool SPLIT>
-LaunchFn pick_out(int out, int kt, int u, bool nt, int r) {
+LaunchFn pick_out(int out, int kt, int u, int nt, int r) {
   if (out == OUT_RUN) return pick_kt<P, W, SPLIT, OUT_RUN>(kt, u, nt, r);
   if (out == OUT_RUNX) return pick_kt<P, W, SPLIT, OUT_RUNX>(kt, u, nt, r);
   if constexpr (std::is_same_v<P, PF64>) return pick_kt<P, W, SPLIT, OUT_RESID>(kt, u, nt, r);
@@ -786,11 +790,11 @@ LaunchFn pick_out(int out, int kt, int u, bool nt, int r) {
 }
 
 template <class P, int W>
-LaunchFn pick_split(bool split, int out, int kt, int u, bool nt, int r) {
+LaunchFn pick_split(bool split, int out, int kt, int u, int nt, int r) {
   return split ? pick_out<P, W, true>(out, kt, u, nt, r) : pick_out<P, W, false>(out, kt, u, nt, r);
 }
 
-LaunchFn pick(int prec, int W, bool split, int out, int kt, int u, bool nt, int r) {
+LaunchFn pick(int prec, int W, bool split, int out, int kt, int u, int nt, int r) {
   // prec: 0 FP64, 1 FP32 compensated, 2 FP32 with FP64 accumulation
   if (prec == 0)
     return W == 64 ? pick_split<PF64, 64>(split, out, kt, u, nt, r)
@@ -1213,7 +1217,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       j0 += ntiles;
       continue;
     }
-    LaunchFn fn = pick(prec, D.W, split, out, kt, cfg.unroll, cfg.nontemporal != 0, r);
+    LaunchFn fn = pick(prec, D.W, split, out, kt, cfg.unroll, cfg.nontemporal, r);
     if (!fn) fail(WK_ERR_ARG, "unsupported kernel configuration");
     const int gpw = D.W == 64 ? r : 2;  // groups per wave
     const int64_t waves = ((int64_t)D.ngroups + gpw - 1) / gpw;

@@ -111,9 +111,11 @@ struct PF32D {
 
 // Steps [i, end) of R groups in lockstep, U slots per group per batch.  All
 // loads of a batch are issued before the first gather waits, so a wave has
-// R*U value+index loads in flight per batch.  Values stream with the
-// nontemporal hint (NT); 4-byte index loads measured faster without it.
-template <class P, int W, int KT, int U, bool NT, int R>
+// R*U value+index loads in flight per batch.  NT 1: values stream with the
+// nontemporal hint (4-byte index loads measured faster without it on banded
+// matrices); NT 2: indices too, which leaves more of L2 to scattered x
+// gathers; NT 0: plain loads.
+template <class P, int W, int KT, int U, int NT, int R>
 __device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
                                                  const typename P::V *const (&vp)[R],
                                                  const int32_t *const (&cp)[R],
@@ -128,8 +130,8 @@ __device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
     for (int r = 0; r < R; r++)
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        c[r][u] = cp[r][(int64_t)(i + u) * W];
-        v[r][u] = ld_stream<NT>(vp[r] + (int64_t)(i + u) * W);
+        c[r][u] = ld_stream<(NT >= 2)>(cp[r] + (int64_t)(i + u) * W);
+        v[r][u] = ld_stream<(NT >= 1)>(vp[r] + (int64_t)(i + u) * W);
       }
     X xv[R][U][KT];
 #pragma unroll
@@ -152,7 +154,7 @@ __device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
 
 // The last, partial batch of one group: steps [i, end) with end - i < U, all
 // loads issued before the first wait (the guards are wave-uniform for W = 64).
-template <class P, int W, int KT, int U, bool NT>
+template <class P, int W, int KT, int U, int NT>
 __device__ __forceinline__ void stream_tail(typename P::Acc (&acc)[KT], const typename P::V *vp,
                                             const int32_t *cp, const typename P::X *xb,
                                             int64_t ldx, uint32_t i, uint32_t end) {
@@ -164,8 +166,8 @@ __device__ __forceinline__ void stream_tail(typename P::Acc (&acc)[KT], const ty
 #pragma unroll
   for (int u = 0; u < U - 1; u++)
     if (i + u < end) {
-      c[u] = cp[(int64_t)(i + u) * W];
-      v[u] = ld_stream<NT>(vp + (int64_t)(i + u) * W);
+      c[u] = ld_stream<(NT >= 2)>(cp + (int64_t)(i + u) * W);
+      v[u] = ld_stream<(NT >= 1)>(vp + (int64_t)(i + u) * W);
     }
   X xv[U][KT];
 #pragma unroll

@@ -44,6 +44,12 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
           c.rows_per_lane = 1;
           c.stage_lds = 0;
           out.push_back(c);
+          if (u == 8 && D.W == 64) {
+            // the index stream nontemporal too (scattered x gathers keep more L2)
+            c.nontemporal = 2;
+            out.push_back(c);
+            c.nontemporal = base.nontemporal;
+          }
           if (D.W == 64 && (size_t)wpb * u * 64 * (vsz + 4) <= 65536) {
             for (int st : {1, 2}) {
               c.stage_lds = st;
@@ -66,6 +72,16 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
           c.unroll = 8;
           c.rows_per_lane = 2;
           out.push_back(c);
+          c.nontemporal = 2;
+          out.push_back(c);
+          c.nontemporal = base.nontemporal;
+          // four groups per wave (scattered columns)
+          c.rows_per_lane = 4;
+          for (int u : {4, 8}) {
+            c.unroll = u;
+            out.push_back(c);
+          }
+          c.unroll = 8;
         }
       } else {
         c.unroll = 8;
@@ -197,6 +213,18 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
       c.waves_per_block = k < 16 ? 1 : 2;
       c.unroll = k < 8 ? 8 : 16;
     }
+  } else if (k == 1 && win_ok && reuse < 1.25) {
+    // scattered columns: nearly every slot of a group reads its own column of
+    // x (a randomly numbered mesh), so each slot costs a separate line fetch
+    // and the rate is set by the gathers in flight; four groups per wave with
+    // four slots each in flight (P1 tets 96^3, randomly permuted, 1.00 slots
+    // per window entry: 96.5 us against 120.8 with one group per wave at
+    // unroll 8, 108.3 with two groups per wave)
+    c.stage_lds = 0;
+    c.rows_per_lane = 4;
+    c.unroll = 4;
+    c.waves_per_block = 2;
+    c.xcd_remap = 1;
   } else if (k == 1 && D.W == 64 && mean_len >= 20.0) {
     // 16-byte batch copies through LDS, next batch in flight during the FMAs
     // (measured best on hex diffusion, 27-entry rows)
