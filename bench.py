@@ -449,6 +449,17 @@ def extra_orderings(torch, dev, steps, warmup, tune=True, flusher=None):
                 row[f"{key}_run_cold_us"] = round(c["cold_ms"] * 1e3, 2)
                 row[f"{key}_run_cold_GBs"] = round(nb / (c["cold_ms"] * 1e-3) / 1e9, 1)
                 row[f"{key}_run_x_cold_GBs"] = round(nb / (cx["cold_ms"] * 1e-3) / 1e9, 1)
+                if th is None:
+                    # the same launch shape with the values not loaded: the
+                    # column stream and the x gathers alone
+                    lc = eng.get_launch().as_dict()
+                    gc = dict(lc, unroll=lc["unroll"] if lc["unroll"] in (4, 8) else 8)
+                    _, msg = time_launches(torch, lambda: eng.gather_ceiling(dx, y, cfg=gc), steps, warmup)
+                    g = warm_cold(torch, lambda: eng.gather_ceiling(dx, y, cfg=gc), steps, flusher)
+                    row[f"{key}_gather_ceiling_us"] = round(msg * 1e3, 2)
+                    row[f"{key}_gather_ceiling_cold_us"] = round(g["cold_ms"] * 1e3, 2)
+                    row[f"{key}_run_frac_of_gather_ceiling"] = round(msg / ms, 3)
+                    row[f"{key}_run_cold_frac_of_gather_ceiling"] = round(g["cold_ms"] / c["cold_ms"], 3)
             del eng
             if th is None:
                 # Kr row sort: each row's x gathers in ascending permuted order
@@ -558,6 +569,9 @@ def run_case(args, torch, dev):
             eng.set_launch(json.loads(args.launch))
         fns = [lambda: eng.launch(x, y)]
         info["launch"] = eng.get_launch().as_dict()
+        if args.gather:
+            fns.append(lambda: eng.gather_ceiling(x, y))
+            info["order"] = "per step: wk_run, wk_engine_gather_ceiling"
     elif name.startswith("batch64"):
         ms = fe.batch_mix(64)
         engs = [engine(structure().scan(mm.nnz, mm.nrows, mm.vals, mm.rowptr, mm.colind), mm.vals,
@@ -687,6 +701,8 @@ def main():
     ap.add_argument("--subset", choices=["all", "windowed", "direct"], default="all",
                     help="batch64 case: all matrices, or those with (without) k > 1 windows")
     ap.add_argument("--batch-stage", type=int, default=5, help="batch64 case: stage_lds")
+    ap.add_argument("--gather", action="store_true",
+                    help="tet96 / tet96_perm cases: also launch the gather ceiling each step")
     ap.add_argument("--launch", default=None,
                     help="tet96 / tet96_perm cases: launch fields to override (JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -475,6 +475,18 @@ int wk_time_run(wk_engine *engine, const wk_launch *cfg, int32_t which, const vo
  * is written only if the XOR of the data hits a fixed value. */
 int wk_stream_read(const void *buf, int64_t bytes, uint32_t *sink, void *stream);
 
+/* Measurement helper, no counterpart in the reference: the forward SpMV's
+ * direct kernel (group width 64, one right-hand side) with the values not
+ * loaded, every slot counting 1.0: it reads the column stream and gathers
+ * x[col] for every slot exactly as wk_run's direct kernel does with the same
+ * unroll (4 or 8), rows_per_lane, waves_per_block and xcd_remap (cfg, or the
+ * engine's launch for WK_OP_RUN at k = 1 when NULL), and writes y = the sum
+ * of x over each row's slots, pads included.  Its duration is the ceiling of
+ * a layout whose x gathers miss the caches (randomly numbered meshes).  FP64
+ * engines; x [ncols], y [nrows] device arrays. */
+int wk_engine_gather_ceiling(wk_engine *engine, const wk_launch *cfg, const double *x_dev,
+                             double *y_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -281,3 +281,26 @@ def test_autotuner_keeps_results_and_stores_choice():
     yk.fill_(float("nan"))
     eng.launch(dev(xk), yk)
     assert np.array_equal(host(yk), O.csr_spmv(m.rowptr, m.colind, m.vals, xk))
+
+
+@pytest.mark.parametrize("cfg", [None, dict(rows_per_lane=1, unroll=8), dict(rows_per_lane=2, unroll=4),
+                                 dict(rows_per_lane=4, unroll=8, waves_per_block=1, xcd_remap=0)])
+def test_gather_ceiling_sums_x_over_slots(cfg):
+    """The gather-ceiling kernel is the direct kernel with every slot's value
+    1.0: the layout SpMV with unit values (pads read column 0)."""
+    say(f"[gpu] gather ceiling {cfg}")
+    m = fe.tet_poisson_3d(40, drop=0.05)
+    x = fe.uniform_x(m.ncols, seed=2006)
+    for variant in (1, 2):
+        if variant == 1:
+            L = O.scan1(m.rowptr, 64)
+            s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+        else:
+            L = O.scan2(m.rowptr, 4, 64, 32)
+            s = structure2().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, 4)
+        _, pc = O.process(L, m.vals, m.colind)
+        ref = O.wpk_spmv(L, np.ones(len(pc)), pc, x)
+        eng = engine(s, m.vals, m.colind)
+        dy = torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+        eng.gather_ceiling(dev(x), dy, cfg=cfg)
+        assert np.array_equal(host(dy), ref), variant

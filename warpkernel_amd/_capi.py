@@ -100,6 +100,7 @@ SIGNATURES = {
     "wk_dvals": (C.c_int, [P, P, I64, P, I64, I32, P, P]),
     "wk_time_run": (C.c_int, [P, C.POINTER(Launch), I32, P, P, I32, I64, I64, P, C.POINTER(C.c_float)]),
     "wk_stream_read": (C.c_int, [P, I64, P, P]),
+    "wk_engine_gather_ceiling": (C.c_int, [P, P, P, P, P]),
     "wk_tune": (C.c_int, [P, I32, C.POINTER(Launch), P, P, I32, I64, I64, I32, P, C.POINTER(Launch),
                           C.POINTER(C.c_float)]),
     "wk_engine_get_launch": (C.c_int, [P, I32, I32, C.POINTER(Launch)]),

@@ -429,6 +429,13 @@ class engine:
         check(lib().wk_engine_x_windows(self._h, C.byref(n)))
         return n.value
 
+    def gather_ceiling(self, x, y, cfg=None, stream=None):
+        """The forward direct kernel without its values (every slot 1.0): the
+        column stream and the x gathers alone, timed as the ceiling of layouts
+        whose gathers miss the caches.  y = per-row sums of x over the slots."""
+        c = self._cfg(capi.WK_OP_RUN, 1, cfg)
+        check(lib().wk_engine_gather_ceiling(self._h, self._ref(c), _dev_ptr(x), _dev_ptr(y), _stream(stream)))
+
     # -- verification (host) ------------------------------------------------
     def verify(self, orig, comp) -> bool:
         """engine::verify (hpp:932-943): relative 1e-5 where |orig| > 1e-14,

@@ -105,6 +105,15 @@ constexpr int batch_wpb() {
   return STAGE == 5 ? 1 : 4;
 }
 
+// Waves per item: at k-tile 8 with windows (stage 5) and one item per block,
+// the block's two waves fill the item's window together and each sums four
+// of the eight right-hand sides (the single-matrix shape for k = 8); one
+// otherwise.
+template <int KT, int STAGE>
+constexpr int batch_wpi() {
+  return KT == 8 && STAGE == 5 ? 2 : 1;
+}
+
 // One group of one matrix for right-hand sides [j0, j0+KT): the summation
 // order of warpKernel / warpKernel2 (every slot by fma onto -0, i.e. first
 // product then fma; the pairwise lane tree for split rows).  With STAGE > 0
@@ -112,9 +121,16 @@ constexpr int batch_wpb() {
 // with STAGE 3 or 5 a windowed matrix reads x from the wave's LDS window (the
 // win_bytes in front of the slice), at STAGE 5 with packed positions where the
 // matrix has them.
-template <class P, int KT, int U, int STAGE>
+//
+// WPI waves per item (k-tiles above 1): wave wib of the block sums right-hand
+// sides [j0 + wib*KW, j0 + (wib+1)*KW) of the k-tile, KW = KT / WPI; with a
+// window the block's waves fill it together (one barrier; the block runs this
+// one item only).
+template <class P, int KT, int U, int STAGE, int WPI = 1>
 __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32_t j0, int32_t lane,
-                                            char *region, int32_t win_bytes) {
+                                            int32_t wib, char *region, int32_t win_bytes) {
+  constexpr int KW = KT / WPI;
+  static_assert(KW * WPI == KT && (WPI == 1 || KT > 1), "waves split the k-tile evenly");
   char *slice = region + win_bytes;
   using V = typename P::V;
   using X = typename P::X;
@@ -126,7 +142,8 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   const V *vbase = static_cast<const V *>(d.vals) + base;
   const int32_t *cbase = d.cols + base;
   const X *xb = static_cast<const X *>(d.x) + j0;
-  Acc acc[KT];
+  const uint32_t t0 = WPI > 1 ? (uint32_t)wib * KW : 0u;
+  Acc acc[KW];
   bool done = false;
   if constexpr (STAGE == 5 && KT > 1) {
     // several right-hand sides: the window holds KT values per column, values
@@ -134,10 +151,12 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
     // alternating), as the single-matrix default
     if (d.windowed) {
       const uint32_t w0 = d.win_off[g], w1 = d.win_off[g + 1];
-      WinFill<X, KT> f;
-      f.cols(d.win_cols + w0, w1 - w0, lane);
-      f.gather(region, d.win_cols + w0, w1 - w0, xb, d.ldx, lane);
-      window_group_regs<P, U, KT, KT>(acc, vbase, d.win_idx + base, region, len, 0u, lane);
+      WinFill<X, KT, 64 * WPI> f;
+      const int32_t first = WPI > 1 ? wib * 64 + lane : lane;
+      f.cols(d.win_cols + w0, w1 - w0, first);
+      f.gather(region, d.win_cols + w0, w1 - w0, xb, d.ldx, first);
+      if constexpr (WPI > 1) __syncthreads();  // the window is the block's
+      window_group_regs<P, U, KT, KW>(acc, vbase, d.win_idx + base, region, len, t0, lane);
       done = true;
     }
   }
@@ -179,46 +198,48 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
   if (!done) {
     (void)slice;
 #pragma unroll
-    for (int j = 0; j < KT; j++) P::init(acc[j]);
+    for (int j = 0; j < KW; j++) P::init(acc[j]);
     const V *const vp[1] = {vbase + lane};
     const int32_t *const cp[1] = {cbase + lane};
-    Acc ar[1][KT];
+    Acc ar[1][KW];
 #pragma unroll
-    for (int j = 0; j < KT; j++) ar[0][j] = acc[j];
-    const uint32_t i = stream_steps<P, 64, KT, U, true, 1>(ar, vp, cp, xb, d.ldx, 0u, len);
-    stream_tail<P, 64, KT, U, true>(ar[0], vp[0], cp[0], xb, d.ldx, i, len);
+    for (int j = 0; j < KW; j++) ar[0][j] = acc[j];
+    const uint32_t i = stream_steps<P, 64, KW, U, true, 1>(ar, vp, cp, xb + t0, d.ldx, 0u, len);
+    stream_tail<P, 64, KW, U, true>(ar[0], vp[0], cp[0], xb + t0, d.ldx, i, len);
 #pragma unroll
-    for (int j = 0; j < KT; j++) acc[j] = ar[0][j];
+    for (int j = 0; j < KW; j++) acc[j] = ar[0][j];
   }
   for (uint32_t o = 1; o < tpr; o <<= 1) {
 #pragma unroll
-    for (int j = 0; j < KT; j++) {
+    for (int j = 0; j < KW; j++) {
       Acc other = P::shfl_down(acc[j], (int)o, 64);
       P::add(acc[j], other);
     }
   }
   const uint32_t rowid = row0 + (uint32_t)lane / tpr;
   if (((uint32_t)lane & (tpr - 1)) != 0 || rowid >= (uint32_t)d.nznrows) return;
-  double *yr = d.y + (int64_t)d.row_map[rowid] * d.ldy + j0;
+  double *yr = d.y + (int64_t)d.row_map[rowid] * d.ldy + j0 + t0;
 #pragma unroll
-  for (int j = 0; j < KT; j++) yr[j] = P::result(acc[j]);
+  for (int j = 0; j < KW; j++) yr[j] = P::result(acc[j]);
 }
 
-template <class P, int KT, int U, int STAGE>
+template <class P, int KT, int U, int STAGE, int WPI>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict__ desc,
                                                     const BatchItem *__restrict__ items,
                                                     const int32_t *__restrict__ wstart,
                                                     int32_t nwaves, int32_t ktiles,
                                                     int32_t win_bytes) {
   // wstart == nullptr: wave w runs item w alone.  Per wave, LDS holds the
-  // window (win_bytes, stage 3 and 5) and then the staging slice.
+  // window (win_bytes, stage 3 and 5) and then the staging slice.  WPI > 1:
+  // block w runs item w (no wstart), its waves sharing one region.
   constexpr int kSlice = batch_slice<P, KT, U, STAGE>();
   extern __shared__ __attribute__((aligned(16))) char lds[];
   const int32_t region = win_bytes + kSlice;
   const int32_t lane = threadIdx.x & 63;
   const int32_t wib = threadIdx.x >> 6;
-  const int32_t w = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * (blockDim.x >> 6) + wib));
-  if (w >= nwaves) return;
+  const int32_t w = __builtin_amdgcn_readfirstlane(
+      WPI > 1 ? (int32_t)blockIdx.x : (int32_t)(blockIdx.x * (blockDim.x >> 6) + wib));
+  if (w >= nwaves) return;  // WPI > 1: the same for the whole block
   const int32_t it0 = wstart ? wstart[w] : w;
   const int32_t it1 = wstart ? wstart[w + 1] : w + 1;
   for (int32_t it = it0; it < it1; it++) {
@@ -226,14 +247,17 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchDesc *__restrict_
     const BatchDesc d = desc[item.m];
     if (item.g < 0) {
       const int64_t pos = (int64_t)d.nznrows + (int64_t)(-1 - item.g) * 64 + lane;
-      if (pos < d.nrows) {
+      if (pos < d.nrows && (WPI == 1 || wib == 0)) {  // WPI > 1: the block's first wave
         double *yr = d.y + (int64_t)d.row_map[pos] * d.ldy;
         for (int32_t j = 0; j < ktiles * KT; j++) yr[j] = 0.0;
       }
       continue;
     }
-    for (int32_t kt = 0; kt < ktiles; kt++)
-      batch_group<P, KT, U, STAGE>(d, item.g, kt * KT, lane, lds + wib * region, win_bytes);
+    for (int32_t kt = 0; kt < ktiles; kt++) {
+      if (WPI > 1 && kt > 0) __syncthreads();  // the previous k-tile's window is read
+      batch_group<P, KT, U, STAGE, WPI>(d, item.g, kt * KT, lane, wib,
+                                        lds + (WPI > 1 ? 0 : wib) * region, win_bytes);
+    }
   }
 }
 
@@ -243,13 +267,22 @@ template <class P, int KT, int STAGE>
 void launch_batch(const wk_batch *B, int32_t ktiles, hipStream_t s) {
   const bool flat = B->nwaves == 0;
   const int64_t waves = flat ? B->nitems : B->nwaves;
+  constexpr int kSlice = batch_slice<P, KT, 8, STAGE>();
+  const int32_t win = STAGE == 3 || STAGE == 5 ? B->win_bytes : 0;
+  constexpr int wpi = batch_wpi<KT, STAGE>();
+  if (wpi > 1 && flat) {
+    // one item per block, its waves sharing the window
+    if (waves > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+    hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE, wpi>), dim3((uint32_t)waves), dim3(wpi * 64),
+                       (size_t)(win + kSlice), s, B->d_desc, B->d_items, nullptr, (int32_t)waves,
+                       ktiles, win);
+    return;
+  }
   constexpr int wpb = batch_wpb<STAGE>();
   const int64_t blocks = (waves + wpb - 1) / wpb;
   if (blocks > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
-  constexpr int kSlice = batch_slice<P, KT, 8, STAGE>();
-  const int32_t win = STAGE == 3 || STAGE == 5 ? B->win_bytes : 0;
   const size_t lds = wpb * (size_t)(win + kSlice);
-  hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE>), dim3((uint32_t)blocks), dim3(wpb * 64), lds, s,
+  hipLaunchKernelGGL((batch_kernel<P, KT, 8, STAGE, 1>), dim3((uint32_t)blocks), dim3(wpb * 64), lds, s,
                      B->d_desc, B->d_items, flat ? nullptr : B->d_wstart, (int32_t)waves, ktiles,
                      win);
 }

@@ -584,6 +584,17 @@ int wk_stream_read(const void *buf, int64_t bytes, uint32_t *sink, void *stream)
   });
 }
 
+int wk_engine_gather_ceiling(wk_engine *E, const wk_launch *cfg, const double *x_dev, double *y_dev,
+                             void *stream) {
+  return guarded([&] {
+    check_engine(E);
+    if (E->dtype != WK_F64) fail(WK_ERR_ARG, "the gather ceiling runs on FP64 engines");
+    if ((!x_dev && E->ncols) || (!y_dev && E->nrows)) fail(WK_ERR_ARG, "NULL vector");
+    launch_gather_ceiling(E->fwd, resolve(E, WK_OP_RUN, 1, cfg), x_dev, y_dev,
+                          static_cast<hipStream_t>(stream));
+  });
+}
+
 int wk_time_run(wk_engine *E, const wk_launch *cfg, int32_t which, const void *x_dev,
                 double *y_dev, int32_t k, int64_t ldx, int64_t ldy, void *stream, float *seconds) {
   return guarded([&] {

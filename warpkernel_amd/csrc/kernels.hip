@@ -546,14 +546,19 @@ __global__ void scatter_bounded_kernel(const T *__restrict__ src, const uint32_t
 // dvals[t] = sum_j dy[row_t][j] * x[col_t][j], CSR order.  With one
 // right-hand side each block takes a tile of kDvTile consecutive entries;
 // tile_row[b] (built with the engine) is the row holding the tile's first
-// entry.  The block's threads walk the rows the tile spans and stamp each
-// entry slot of an LDS array with its row, so every entry then finds its row
-// with one conflict-free LDS read.  Column reads and dvals writes are
-// coalesced.  The loads that do not need the rows (columns and x) are issued
-// before the stamping, so their latency overlaps the tile_row -> rowptr
-// chain.  Reading a stored row per entry instead measured the same at k = 1
-// (it adds 4 bytes per entry); with several right-hand sides it is faster,
-// and dvals_coo_kernel does that.
+// entry.  Each thread loads its entries' columns and issues their x gathers
+// first; then the block's threads walk the rows the tile spans and stamp
+// each entry slot of an LDS array with its row's dy value, so after one
+// barrier every entry finds its dy with one conflict-free LDS read.  The x
+// gathers are in flight during the tile_row -> rowptr -> dy chain, and no
+// global load follows the barrier.  Column reads and dvals writes are
+// coalesced, one entry per lane (consecutive lanes, consecutive entries: a
+// wave's x gathers fall on the fewest lines; four entries per lane with
+// 16-byte loads and stores measured 13% slower on hex diffusion 48^3).
+// Stamped row numbers with a dy gather after the barrier measured 5% slower
+// (11.0 against 10.45 us at 48^3), and a stored row per entry slower still
+// (it adds 4 bytes per entry); with several right-hand sides the stored rows
+// pay, and dvals_coo_kernel uses them.
 constexpr int kDvTile = 2048;
 constexpr int kDvThreads = 256;
 
@@ -569,10 +574,10 @@ struct DvArgs {
 };
 
 // One right-hand side (several go through dvals_coo_kernel), a tile of M
-// table tiles (M * kDvTile entries, srow as many): its rows lie between the
+// table tiles (M * kDvTile entries, sdy as many): its rows lie between the
 // table rows of its first tile and of the tile after its last.
 template <int M>
-__device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_t *srow) {
+__device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, double *sdy) {
   constexpr int TILE = M * kDvTile;
   constexpr int PER = TILE / kDvThreads;
   const int64_t e0 = (int64_t)tile * TILE;
@@ -587,28 +592,22 @@ __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, int32_
     const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
     c[q] = e < e1 ? d.colind[e] : 0;
   }
+  double xv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) xv[q] = d.x[(int64_t)c[q] * d.ldx];
   // rows of the tile lie in [r0, r1]: the next tile starts in row r1 or later
   for (int32_t r = r0 + (int32_t)threadIdx.x; r <= r1; r += kDvThreads) {
     const int64_t a = d.rowptr[r] > e0 ? d.rowptr[r] : e0;
     const int64_t b = d.rowptr[r + 1] < e1 ? d.rowptr[r + 1] : e1;
-    for (int64_t e = a; e < b; e++) srow[e - e0] = r;
+    if (a >= b) continue;
+    const double dyr = d.dy[(int64_t)r * d.lddy];
+    for (int64_t e = a; e < b; e++) sdy[e - e0] = dyr;
   }
-  {
-    double xv[PER];
+  __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PER; q++) xv[q] = d.x[(int64_t)c[q] * d.ldx];
-    __syncthreads();
-    double dvv[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
-      dvv[q] = d.dy[(int64_t)(e < e1 ? srow[e - e0] : r0) * d.lddy];
-    }
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
-      if (e < e1) d.dv[e] = dvv[q] * xv[q];
-    }
+  for (int q = 0; q < PER; q++) {
+    const int64_t e = e0 + threadIdx.x + (int64_t)q * kDvThreads;
+    if (e < e1) d.dv[e] = sdy[e - e0] * xv[q];
   }
 }
 
@@ -664,18 +663,18 @@ __global__ __launch_bounds__(kDvThreads) void dvals_coo_kernel(DvArgs d, const i
 }
 
 __global__ __launch_bounds__(kDvThreads) void dvals_kernel(DvArgs d) {
-  __shared__ int32_t srow[kDvTile];
-  dvals_tile<1>(d, (int32_t)blockIdx.x, srow);
+  __shared__ double sdy[kDvTile];
+  dvals_tile<1>(d, (int32_t)blockIdx.x, sdy);
 }
 
-// Tiles of the combined backward: two table tiles (4096 entries) each, which
-// measured faster there (dx and dvals 19.05 -> 18.65 us on hex diffusion 48^3)
-// and slower alone (dvals 11.1 -> 12.7 us).
-constexpr int kBwdTiles = 2;
+// Tiles of the combined backward: one table tile (2048 entries, 16 KB of dy
+// stamps) each.  Two tiles per block (32 KB) leave fewer blocks resident:
+// hex diffusion 48^3 dx and dvals 21.2 us against 18.6 with one.
+constexpr int kBwdTiles = 1;
 
 // dx = A^T dy and dvals in one launch, for layouts of short groups: the
 // adjoint blocks (the direct body, one right-hand side, no LDS) and the dvals
-// tiles (kBwdTiles x 8 KB = 16 KB of LDS) are spread evenly over the grid, so one kind's
+// tiles (kBwdTiles x 16 KB of LDS) are spread evenly over the grid, so one kind's
 // latency-bound phases overlap the other's streaming and the launch and tail
 // costs are paid once.  Block b is dvals tile floor(b*ndv/T) when that count
 // steps at b, else adjoint block b - floor(b*ndv/T), T = nspmv + ndv.
@@ -719,13 +718,13 @@ template <bool SPLIT>
 __global__ __launch_bounds__(kDvThreads) void backward_kernel(SpmvArgs a, int32_t nwork,
                                                               int32_t nspmv, DvArgs d,
                                                               int32_t ndv) {
-  __shared__ int32_t srow[kBwdTiles * kDvTile];
+  __shared__ double sdy[kBwdTiles * kDvTile];
   const int64_t T = (int64_t)nspmv + ndv;
   const int64_t b = blockIdx.x;
   const int64_t before = b * ndv / T;
   const int64_t upto = (b + 1) * ndv / T;
   if (upto > before)
-    dvals_tile<kBwdTiles>(d, (int32_t)before, srow);
+    dvals_tile<kBwdTiles>(d, (int32_t)before, sdy);
   else
     direct_block<SPLIT>(a, (int32_t)(b - before), nwork);
 }
@@ -1037,6 +1036,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
                  hipStream_t stream) {
   if (k < 1) fail(WK_ERR_ARG, "k must be >= 1");
   if (cfg.stage_lds < 0 || cfg.stage_lds > 6) fail(WK_ERR_ARG, "stage_lds must be 0 to 6");
+  if (cfg.nontemporal < 0 || cfg.nontemporal > 2) fail(WK_ERR_ARG, "nontemporal must be 0, 1 or 2");
   const int wpb = cfg.waves_per_block;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) fail(WK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
   if (cfg.k_tile < 1 || cfg.k_tile > 32 || (cfg.k_tile & (cfg.k_tile - 1)))
@@ -1235,6 +1235,53 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     hip_check(hipGetLastError(), "spmv launch");
     j0 += ntiles * kt;
   }
+}
+
+// The direct kernel with its values not loaded (NT 3), as launch_spmv runs it
+// at one right-hand side: the time of the index stream and the x gathers
+// alone, the ceiling of a layout whose gathers miss the caches.
+template <bool SPLIT>
+LaunchFn pick_gather(int u, int r) {
+  if (u == 4 && r == 1) return &launch_one<PF64, 64, SPLIT, OUT_RUN, 1, 4, 3, 1>;
+  if (u == 8 && r == 1) return &launch_one<PF64, 64, SPLIT, OUT_RUN, 1, 8, 3, 1>;
+  if (u == 4 && r == 2) return &launch_one<PF64, 64, SPLIT, OUT_RUN, 1, 4, 3, 2>;
+  if (u == 8 && r == 2) return &launch_one<PF64, 64, SPLIT, OUT_RUN, 1, 8, 3, 2>;
+  if (u == 4 && r == 4) return &launch_one<PF64, 64, SPLIT, OUT_RUN, 1, 4, 3, 4>;
+  if (u == 8 && r == 4) return &launch_one<PF64, 64, SPLIT, OUT_RUN, 1, 8, 3, 4>;
+  return nullptr;
+}
+
+void launch_gather_ceiling(const DevLayout &D, const wk_launch &cfg, const double *x, double *y,
+                           hipStream_t stream) {
+  if (D.W != 64) fail(WK_ERR_ARG, "the gather ceiling needs group width 64");
+  const int wpb = cfg.waves_per_block;
+  if (wpb < 1 || wpb > kMaxThreads / 64) fail(WK_ERR_ARG, "waves_per_block out of range");
+  const int r = cfg.rows_per_lane;
+  LaunchFn fn = D.variant == 2 ? pick_gather<true>(cfg.unroll, r) : pick_gather<false>(cfg.unroll, r);
+  if (!fn) fail(WK_ERR_ARG, "gather ceiling: unroll 4 or 8 and rows_per_lane 1, 2 or 4");
+  SpmvArgs a{};
+  a.vals = D.vals;
+  a.cols = D.cols;
+  a.row_map = D.row_map;
+  a.warp_offsets = D.warp_offsets;
+  a.max_nz_rows = D.max_nz_rows;
+  a.tpr = D.tpr;
+  a.row_off = D.row_off;
+  a.ldx = a.ldy = a.ldb = 1;
+  a.ngroups = D.ngroups;
+  a.nznrows = D.nznrows;
+  a.nrows = D.nrows;
+  a.x = x;
+  a.y = y;
+  a.k = 1;
+  const int64_t waves = ((int64_t)D.ngroups + r - 1) / r;
+  const int64_t nwork = (waves + wpb - 1) / wpb;
+  const int64_t nfill = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+  if (nwork + nfill > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+  if (nwork + nfill > 0)
+    fn(a, dim3((uint32_t)(nwork + nfill)), dim3(wpb * 64), 0, (int32_t)nwork, 1,
+       cfg.xcd_remap ? 1 : 0, stream);
+  hip_check(hipGetLastError(), "gather ceiling launch");
 }
 
 void launch_stream_read(const void *p, int64_t bytes, uint32_t *sink, hipStream_t stream) {

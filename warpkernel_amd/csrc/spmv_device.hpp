@@ -114,7 +114,8 @@ struct PF32D {
 // R*U value+index loads in flight per batch.  NT 1: values stream with the
 // nontemporal hint (4-byte index loads measured faster without it on banded
 // matrices); NT 2: indices too, which leaves more of L2 to scattered x
-// gathers; NT 0: plain loads.
+// gathers; NT 0: plain loads; NT 3: as 2 without the values (each slot
+// counts 1.0), which times the index stream and the x gathers alone.
 template <class P, int W, int KT, int U, int NT, int R>
 __device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
                                                  const typename P::V *const (&vp)[R],
@@ -131,7 +132,7 @@ __device__ __forceinline__ uint32_t stream_steps(typename P::Acc (&acc)[R][KT],
 #pragma unroll
       for (int u = 0; u < U; u++) {
         c[r][u] = ld_stream<(NT >= 2)>(cp[r] + (int64_t)(i + u) * W);
-        v[r][u] = ld_stream<(NT >= 1)>(vp[r] + (int64_t)(i + u) * W);
+        v[r][u] = NT == 3 ? V(1) : ld_stream<(NT >= 1)>(vp[r] + (int64_t)(i + u) * W);
       }
     X xv[R][U][KT];
 #pragma unroll
@@ -167,7 +168,7 @@ __device__ __forceinline__ void stream_tail(typename P::Acc (&acc)[KT], const ty
   for (int u = 0; u < U - 1; u++)
     if (i + u < end) {
       c[u] = ld_stream<(NT >= 2)>(cp + (int64_t)(i + u) * W);
-      v[u] = ld_stream<(NT >= 1)>(vp + (int64_t)(i + u) * W);
+      v[u] = NT == 3 ? V(1) : ld_stream<(NT >= 1)>(vp + (int64_t)(i + u) * W);
     }
   X xv[U][KT];
 #pragma unroll

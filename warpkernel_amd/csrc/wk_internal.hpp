@@ -174,6 +174,8 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
 void launch_scatter_values(const DevLayout &D, int32_t dtype, int64_t nnz, const void *vals_csr,
                            hipStream_t stream);
 void launch_stream_read(const void *p, int64_t bytes, uint32_t *sink, hipStream_t stream);
+void launch_gather_ceiling(const DevLayout &D, const wk_launch &cfg, const double *x, double *y,
+                           hipStream_t stream);
 // dst[index[i]] = src[i] for i < n with index[i] < dst_count (others skipped);
 // elements of elem_bytes (4 or 8) bytes.
 void launch_scatter(const void *src, const uint32_t *index, void *dst, int64_t n, int64_t dst_count,
