@@ -523,7 +523,7 @@ def run_case(args, torch, dev):
     """One kernel case, `--steps` launches after two warm-up launches: the
     workload a rocprofv3 counter or trace run attributes.  Prints one JSON
     line naming the case and the launch shape."""
-    from warpkernel_amd import batch, engine, fe, structure, structure2
+    from warpkernel_amd import _capi as capi, batch, engine, fe, structure, structure2
     k = args.k
     name = args.case
     info = {"case": name, "k": k}
@@ -580,7 +580,7 @@ def run_case(args, torch, dev):
             # windowed: matrices whose window entries serve >= 3 slots each
             # (the k > 1 window rule); direct: the others
             def windowed(e):
-                total = int(e.download(21).size)  # entries over all windows
+                total = int(e.download(capi.WK_ENGINE_WIN_COLS).size)  # entries over all windows
                 return total > 0 and e.kernel.allocate_nz >= 3.0 * total
             keep = [windowed(e) == (args.subset == "windowed") for e in engs]
             ms = [mm for mm, kk in zip(ms, keep) if kk]

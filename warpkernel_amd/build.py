@@ -25,7 +25,7 @@ ARCH = "gfx950"
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-Wall", "-Werror=shadow", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread"]
 

@@ -402,7 +402,11 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_winpk_kernel(SpmvArgs a, Win
 // Per lane and right-hand side the order is the other kernels' (every slot by
 // fma onto -0, pads included), so results are bit-identical.  A pass reuses
 // the window of the previous one: the wave's LDS operations complete in
-// order, so the fill's writes follow the previous pass's reads.
+// order, so the fill's writes follow the previous pass's reads.  All of a
+// group's passes stay in one wave: dealing them to two to eight waves (each
+// loading the group's values) measured slower at every size (hex elasticity
+// 32^3 k=32: 99.6 us in one wave, 111.4 to 214 split; 64^3 k=32: 803
+// against 860 to 1746).
 template <class P, bool SPLIT, int OUT, int NB>
 __global__ __launch_bounds__(kMaxThreads) void spmv_winres_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
                                                                   int32_t ntiles, int32_t remap) {
@@ -1135,16 +1139,16 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         w.win_bytes = (int32_t)win_bytes;
         w.pk = D.win_pk;
         w.pk_off = D.win_pk_off;
-        const int64_t nb = ((int64_t)D.ngroups + wpb - 1) / wpb;
+        const int64_t nwork = ((int64_t)D.ngroups + wpb - 1) / wpb;
         const int64_t nf = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
-        if (nb + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+        if (nwork + nf > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
         for (int32_t t = 0; t < ntiles; t++) {
           a.x = static_cast<const char *>(x) + (size_t)(j0 + t) * xsz;
           a.y = y + j0 + t;
           a.b = b ? b + j0 + t : nullptr;
           a.k = 1;
-          if (nb + nf > 0)
-            wf(a, w, dim3((uint32_t)(nb + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nb,
+          if (nwork + nf > 0)
+            wf(a, w, dim3((uint32_t)(nwork + nf)), dim3(wpb * 64), std::max(need, lds), (int32_t)nwork,
                cfg.xcd_remap ? 1 : 0, stream);
           hip_check(hipGetLastError(), "spmv launch");
         }
