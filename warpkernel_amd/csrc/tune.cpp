@@ -198,11 +198,15 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         c.k_tile = 8;
         c.waves_per_block = 1;
       }
-      if (D.win_pk && D.pk_longest <= 84 && mean_len >= 40.0 &&
-          (k >= 32 || (k >= 16 && D.ngroups < 4096))) {
+      if ((D.win_pk && D.pk_longest <= 84 && mean_len >= 40.0 &&
+           (k >= 32 || (k >= 16 && D.ngroups < 4096))) ||
+          (D.win_pk && D.pk_longest <= 36 && mean_len >= 20.0 && k >= 8)) {
         // the group's values in registers across 4-wide passes (stage 6):
         // elasticity 32^3 k=16 58.3 against 61.1 us, k=32 99.9 against 111.6;
-        // 64^3 k=32 803 against 819, but k=16 446 against 423
+        // 64^3 k=32 803 against 819, but k=16 446 against 423.  Groups of at
+        // most 36 steps from k = 8: hex diffusion 48^3 (27-entry rows) k=8
+        // 15.8 against 17.0 us, k=16 35.5 against 37.3, k=32 65.7 against
+        // 68.1; P1 tets (15-entry rows) lose (k=16 69.5 against 68.5)
         c.stage_lds = 6;
         c.k_tile = 8;
         c.waves_per_block = 1;
