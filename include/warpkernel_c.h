@@ -428,8 +428,9 @@ typedef struct wk_batch wk_batch;
  * use direct loads.  With several
  * right-hand sides, stage_lds 5 gives the matrices whose window entries serve
  * at least 3 slots windows of k-tile values per column (values and positions
- * straight into registers), the others direct loads; every other stage_lds
- * means direct loads. */
+ * straight into registers; at k-tile 8 with waves_per_cu 0, two waves per
+ * item share its window, each summing four right-hand sides), the others
+ * direct loads; every other stage_lds means direct loads. */
 int wk_batch_create(wk_engine *const *engines, int32_t count, int32_t waves_per_cu,
                     int32_t stage_lds, wk_batch **out);
 
