@@ -304,3 +304,26 @@ def test_gather_ceiling_sums_x_over_slots(cfg):
         dy = torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda")
         eng.gather_ceiling(dev(x), dy, cfg=cfg)
         assert np.array_equal(host(dy), ref), variant
+
+
+@pytest.mark.parametrize("name", ["hex_laplace_3d_48", "tet_poisson_3d_96"])
+@pytest.mark.parametrize("k", [8, 16, 32])
+def test_multi_rhs_defaults_on_short_rows_bitexact(name, k):
+    """The layout defaults at several right-hand sides on short rows (27-entry hex
+    diffusion rows take the values-in-registers kernel from k = 8; 15-entry tet
+    rows keep the register-fed windows): bit-exact against the CSR oracle
+    column by column, run and run_x."""
+    say(f"[gpu] multi-RHS default {name} k={k}")
+    m = matrix(name)
+    x = fe.uniform_x(m.ncols, k, seed=2400 + k).reshape(m.ncols, k)
+    ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    s = structure().scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind, kr=True)
+    y = torch.full((m.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
+    eng.launch(dev(x), y)
+    assert np.array_equal(host(y), ref), eng.get_launch(k=k).as_dict()
+    xr = np.empty_like(x)
+    xr[np.asarray(s.row_map_inv)] = x
+    y.fill_(float("nan"))
+    eng.launch_x(dev(xr), y)
+    assert np.array_equal(host(y), ref[np.asarray(s.row_map)])
