@@ -599,13 +599,19 @@ __device__ __forceinline__ void dvals_tile(const DvArgs &d, int32_t tile, double
   double xv[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) xv[q] = d.x[(int64_t)c[q] * d.ldx];
-  // rows of the tile lie in [r0, r1]: the next tile starts in row r1 or later
-  for (int32_t r = r0 + (int32_t)threadIdx.x; r <= r1; r += kDvThreads) {
+  // rows of the tile lie in [r0, r1]: the next tile starts in row r1 or later.
+  // Each row is stamped by P = kDvThreads / rows threads (at least 1), each
+  // writing every P-th entry, so long rows do not serialise on one thread
+  // (hex elasticity 32^3, 81-entry rows: 22.2 -> 20.1 us)
+  const int32_t nr = r1 - r0 + 1;
+  const int32_t P = nr >= kDvThreads ? 1 : kDvThreads / nr;
+  for (int32_t idx = (int32_t)threadIdx.x; idx < nr * P; idx += kDvThreads) {
+    const int32_t r = r0 + idx / P, part = idx % P;
     const int64_t a = d.rowptr[r] > e0 ? d.rowptr[r] : e0;
     const int64_t b = d.rowptr[r + 1] < e1 ? d.rowptr[r + 1] : e1;
-    if (a >= b) continue;
+    if (a + part >= b) continue;
     const double dyr = d.dy[(int64_t)r * d.lddy];
-    for (int64_t e = a; e < b; e++) sdy[e - e0] = dyr;
+    for (int64_t e = a + part; e < b; e += P) sdy[e - e0] = dyr;
   }
   __syncthreads();
 #pragma unroll
