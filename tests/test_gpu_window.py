@@ -286,3 +286,24 @@ def test_multi_rhs_residual_and_strided_windows(k):
         eng.residual(bd[:, :k], xd[:, :k], rw[:, :k], cfg=cfg)
         assert np.array_equal(host(rw)[:, :k], b - ax), cfg
     eng.close()
+
+
+@pytest.mark.parametrize("k", [16, 32])
+def test_multi_rhs_columns_match_single_rhs_at_full_size(k):
+    """Hex elasticity 64^3 (the headline matrix): every right-hand side of a
+    k-wide launch (default shape: windows of k-tile values, or the values held
+    in registers across passes) has the same per-row order as a one-column
+    launch, so each column equals the k = 1 result bit for bit."""
+    say(f"[gpu] multi-RHS columns vs k=1, hex_elastic_3d_64, k={k}")
+    m = fe.CONFIGS["hex_elastic_3d_64"]()
+    s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind)
+    x = fe.uniform_x(m.ncols, k, seed=4500 + k).reshape(m.ncols, k)
+    y = torch.full((m.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
+    eng.launch(dev(x), y)
+    yk = host(y)
+    y1 = torch.full((m.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    for j in (0, 3, k // 2, k - 1):
+        eng.launch(dev(np.ascontiguousarray(x[:, j])), y1)
+        assert np.array_equal(yk[:, j], host(y1)), (j, eng.get_launch(k=k).as_dict())
+    eng.close()
