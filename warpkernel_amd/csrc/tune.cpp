@@ -217,13 +217,15 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
       c.waves_per_block = k < 16 ? 1 : 2;
       c.unroll = k < 8 ? 8 : 16;
     }
-  } else if (k == 1 && win_ok && reuse < 1.25) {
+  } else if (k == 1 && win_ok && reuse < 1.25 && D.variant == 1) {
     // scattered columns: nearly every slot of a group reads its own column of
     // x (a randomly numbered mesh), so each slot costs a separate line fetch
     // and the rate is set by the gathers in flight; four groups per wave with
     // four slots each in flight (P1 tets 96^3, randomly permuted, 1.00 slots
     // per window entry: 96.5 us against 120.8 with one group per wave at
-    // unroll 8, 108.3 with two groups per wave)
+    // unroll 8, 108.3 with two groups per wave; FP32 values 76.5 against
+    // 77.2).  WPK2 layouts, whose split groups are a few steps long, keep one
+    // group per wave (threshold 5: 129.2 us against 132.8)
     c.stage_lds = 0;
     c.rows_per_lane = 4;
     c.unroll = 4;
