@@ -140,10 +140,13 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
   // (k = 1 with packed positions from 3 slots per entry: hex diffusion 48^3,
   // 3.4, forward and adjoint 7.16 us against 7.76 staged; P1 tets, 2.1, and
   // the 2D Laplacian, 2.9, stay faster without windows)
+  // (WPK2 layouts from k = 8 at 2.5: hex diffusion 48^3 at threshold 5, 2.74
+  // slots per entry, k=8 25.0 us against 32.3 direct)
   const bool windows =
       win_ok && (k == 1 ? reuse >= (D.win_pk ? 3.0 : 4.5)
                         : reuse >= 3.0 || (reuse >= 2.0 && (k >= 16 || (dtype != WK_F64 &&
-                                                                         D.ngroups >= 1024))));
+                                                                         D.ngroups >= 1024))) ||
+                              (D.variant == 2 && k >= 8 && reuse >= 2.5));
   if (k == 1 && windows) {
     // 10 bytes per slot, x from the group's window in LDS; one wave per
     // block, a contiguous range of groups per XCD, and 16-step batches for
@@ -200,7 +203,7 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
       }
       if ((D.win_pk && D.pk_longest <= 84 && mean_len >= 40.0 &&
            (k >= 32 || (k >= 16 && D.ngroups < 4096))) ||
-          (D.win_pk && D.pk_longest <= 36 && mean_len >= 20.0 && k >= 8)) {
+          (D.variant == 1 && D.win_pk && D.pk_longest <= 36 && mean_len >= 20.0 && k >= 8)) {
         // the group's values in registers across 4-wide passes (stage 6):
         // elasticity 32^3 k=16 58.3 against 61.1 us, k=32 99.9 against 111.6;
         // 64^3 k=32 803 against 819, but k=16 446 against 423.  Groups of at
@@ -211,10 +214,40 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         c.k_tile = 8;
         c.waves_per_block = 1;
       }
+      if (D.variant == 2 && k >= 8) {
+        // WPK2's split groups are a few steps long: one wave per window and
+        // 16-step batches (hex elasticity 32^3 at threshold 27, k=8 28.5 us
+        // against 32.8, k=16 59.2 against 64.9; hex diffusion 48^3 at
+        // threshold 5, k=16 52.4 against 76.0; P1 tets 96^3 at threshold 5,
+        // k=16 256 against 323), and from k = 32 the values in registers
+        // (hex diffusion 95.0 against 145, tets 535 against 661; from 4,096
+        // groups: the 2D Laplacian 128^2 at threshold 5, 521 groups, k=32
+        // 15.1 against 9.3)
+        c.stage_lds = 4;
+        c.k_tile = 8;
+        c.waves_per_block = 1;
+        c.unroll = 16;
+        if (k >= 32 && D.win_pk && D.pk_longest <= 84 && D.ngroups >= 4096) {
+          c.stage_lds = 6;
+          c.unroll = 8;
+        }
+      }
+    } else if (mean_len < 20.0 && k >= 16 && D.ngroups >= 4096) {
+      // FP32 values on short rows from k = 16: the direct kernel with wide
+      // k-tiles outruns the windows (P1 tets 96^3: k=16 111 us against 120,
+      // k=32 279 against 329), except on small layouts, whose few waves it
+      // leaves idle (the 2D Laplacian 128^2, 261 groups: k=32 11.6 against 6.7)
+      c.stage_lds = 0;
+      c.unroll = 8;
+      c.k_tile = std::min(32, pow2_at_most(k));
+      c.waves_per_block = k >= 32 ? 1 : 2;
     } else {
+      // (below k = 16 rows shorter than 40 share a window between two
+      // waves: hex diffusion 48^3 k=4 7.6 us against 9.0, P1 tets k=8 53.5
+      // against 66.0; hex elasticity keeps one, 22.5 against 22.7)
       c.stage_lds = 4;
       c.k_tile = std::min(8, pow2_at_most(k));
-      c.waves_per_block = k < 16 ? 1 : 2;
+      c.waves_per_block = k < 16 && mean_len >= 40.0 ? 1 : 2;
       c.unroll = k < 8 ? 8 : 16;
     }
   } else if (k == 1 && win_ok && reuse < 1.25 && D.variant == 1) {
@@ -245,6 +278,16 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     c.waves_per_block = 2;
     c.xcd_remap = 1;
     c.k_tile = std::min(8, pow2_at_most(std::max(1, k)));
+    if (win_ok && reuse < 1.25 && k >= 16) {
+      // scattered columns from k = 16: eight waves per block (randomly
+      // numbered P1 tets, k=16: FP64 343 us against 375; FP32 with k-tiles
+      // of k and no remap 240 against 355, k=32 315 against 517)
+      c.waves_per_block = 8;
+      if (dtype != WK_F64) {
+        c.k_tile = std::min(32, pow2_at_most(k));
+        c.xcd_remap = 0;
+      }
+    }
   }
   c.k_tile = std::min(c.k_tile, pow2_at_most(std::max(1, k)));  // as the launcher uses it
   return c;
