@@ -146,7 +146,9 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
       win_ok && (k == 1 ? reuse >= (D.win_pk ? 3.0 : 4.5)
                         : reuse >= 3.0 || (reuse >= 2.0 && (k >= 16 || (dtype != WK_F64 &&
                                                                          D.ngroups >= 1024))) ||
-                              (D.variant == 2 && k >= 8 && reuse >= 2.5));
+                              (D.variant == 2 && k >= 8 && reuse >= 2.5) ||
+                              (dtype == WK_F64 && D.variant == 1 && k >= 4 && reuse >= 2.0 &&
+                               mean_len < 20.0 && D.ngroups >= 4096));
   if (k == 1 && windows) {
     // 10 bytes per slot, x from the group's window in LDS; one wave per
     // block, a contiguous range of groups per XCD, and 16-step batches for
@@ -213,6 +215,28 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         c.stage_lds = 6;
         c.k_tile = 8;
         c.waves_per_block = 1;
+      }
+      if (D.variant == 1 && mean_len < 20.0 && D.ngroups >= 4096) {
+        // short rows with 2-3 slots per window entry (P1 tets 96^3, 15-entry
+        // rows, 2.1): 16-step batches, one wave per k-tile-4 window below
+        // k = 8 (k=4 44.5 us against 49.6 direct), the values in registers
+        // at k = 8 (83.5 against 89.5), k-tile 8 over two waves from k = 16
+        // (k=16 215.5 against 229.6, k=32 452.5 against 475.9)
+        c.unroll = 16;
+        if (k < 8) {
+          c.stage_lds = 4;
+          c.k_tile = std::min(4, pow2_at_most(k));
+          c.waves_per_block = 1;
+        } else if (k < 16 && D.win_pk && D.pk_longest <= 36) {
+          c.stage_lds = 6;
+          c.k_tile = 8;
+          c.waves_per_block = 1;
+          c.unroll = 8;
+        } else {
+          c.stage_lds = 4;
+          c.k_tile = 8;
+          c.waves_per_block = 2;
+        }
       }
       if (D.variant == 2 && k >= 8) {
         // WPK2's split groups are a few steps long: one wave per window and
