@@ -246,7 +246,9 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         // k=16 256 against 323), and from k = 32 the values in registers
         // (hex diffusion 95.0 against 145, tets 535 against 661; from 4,096
         // groups: the 2D Laplacian 128^2 at threshold 5, 521 groups, k=32
-        // 15.1 against 9.3)
+        // 15.1 against 9.3), four waves to a block (tets 535.5 against 586.0
+        // with one, elasticity 64^3 846.8 against 872.1, 32^3 112.3 against
+        // 115.6; hex diffusion 96.1 against 94.7)
         c.stage_lds = 4;
         c.k_tile = 8;
         c.waves_per_block = 1;
@@ -254,6 +256,7 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
         if (k >= 32 && D.win_pk && D.pk_longest <= 84 && D.ngroups >= 4096) {
           c.stage_lds = 6;
           c.unroll = 8;
+          c.waves_per_block = 4;
         }
       }
     } else if (mean_len < 20.0 && k >= 16 && D.ngroups >= 4096) {
