@@ -271,6 +271,15 @@ int wk_engine_create_device(int32_t variant, int32_t threshold, int32_t max_tpr,
 #define WK_ENGINE_WIN_OFF 20  /* uint32 [nwarps+1]     window offsets per group */
 #define WK_ENGINE_WIN_COLS 21 /* int32 [WIN_OFF[nwarps]] window columns, ascending per group */
 #define WK_ENGINE_WIN_COLS_KR 22 /* int32, the window columns in Kr numbering */
+/* node windows (stage_lds 7), present where every triple of a lane's steps
+ * reads columns 3n, 3n+1, 3n+2 or one window position three times: */
+#define WK_ENGINE_ND_COLS 23  /* int32 [WIN_OFF[nwarps]] from WIN_OFF[g]: group g's ND_CNT[g]
+                                 entries, e >= 0 for x[e..e+2], e < 0 for x[-e-1] thrice */
+#define WK_ENGINE_ND_CNT 24   /* uint32 [nwarps] entries per group */
+#define WK_ENGINE_ND_OFF 25   /* uint32 [nwarps+1] first position word of each group */
+#define WK_ENGINE_ND_POS 26   /* uint32 [ND_OFF[nwarps] * 64] word w of lane l of group g at
+                                 (ND_OFF[g] + w) * 64 + l: the 10-bit entries of steps
+                                 9w+3f .. 9w+3f+2 in bits 10f .. 10f+9 */
 
 /* Copies one of the engine's device arrays of the forward layout to the host.
  * host_dst NULL: *count receives the array's length (0 for an array the engine
@@ -326,7 +335,12 @@ typedef struct wk_launch {
                               right-hand sides through a window of that
                               many values per column; the rest of k (fewer
                               than one tile), and engines it does not fit,
-                              run as 4; otherwise ignored */
+                              run as 4; 7 (k_tile 1, engines with node
+                              windows, run and the residual): as 5, with a
+                              window of node entries (three x values at
+                              columns 3n .. 3n+2, or one column thrice for
+                              pads) and one 10-bit entry index per three
+                              steps, 27-step batches; otherwise as 5 */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);
@@ -394,8 +408,10 @@ int wk_dvals(wk_engine *engine, const double *dy_dev, int64_t lddy, const double
  * every candidate launch configuration of operation `op` on this engine with
  * the given device vectors (k right-hand sides, reps launches each, HIP
  * events on `stream`), times the three fastest twice more (each keeps its best
- * mean), stores the fastest in the engine for (op, k) and returns it with that
- * mean time.  Only performance knobs vary; base (may be
+ * mean) beside the incumbent (base, or the engine's launch for (op, k)), and
+ * stores the fastest in the engine for (op, k), or the incumbent unless the
+ * fastest beats it by more than 1%; returns it with its mean time.  Only
+ * performance knobs vary; base (may be
  * NULL) fixes the numerics (accumulate).  Results are identical for every
  * candidate.  y_dev is overwritten. */
 int wk_tune(wk_engine *engine, int32_t op, const wk_launch *base, const void *x_dev,

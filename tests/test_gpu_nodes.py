@@ -1,0 +1,211 @@
+"""Node windows (stage_lds = 7) on the GPU through the C ABI.
+
+Where every triple of a lane's steps reads columns 3n, 3n+1, 3n+2 (3D
+elasticity: three unknowns per node, each row's entries in whole nodes) or
+one window position three times (pads), the engine also holds each group's
+window as node entries of three x values, and per lane one 10-bit entry index
+per three steps.  Structure: every slot, decoded through the node window,
+names its padded column (pads read column 0).  Products: the node kernel sums
+the same slots in the same order as the packed-window kernel (stage_lds 5),
+so y is bit-identical to it in every precision and, in FP64, to the oracle.
+Layouts without node windows, run_x and k-tiles above 1 run as stage 5."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from warpkernel_amd import engine, fe, structure, structure2
+from warpkernel_amd import _capi as capi
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+PK = dict(stage_lds=5, nontemporal=1, rows_per_lane=1, k_tile=1, unroll=16)
+ND = dict(PK, stage_lds=7)
+
+_CACHE = {}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def nan_like(n, k=1):
+    return torch.full((n,) if k == 1 else (n, k), float("nan"), dtype=torch.float64, device="cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def matrix(name):
+    if name not in _CACHE:
+        _CACHE.clear()
+        _CACHE[name] = fe.CONFIGS[name]() if name in fe.CONFIGS else fe.hex_elastic_3d(int(name))
+    return _CACHE[name]
+
+
+def layout(m, variant, th=27):
+    """The engine's scan and the oracle's layout of the same matrix."""
+    if variant == 1:
+        return structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind), O.scan1(m.rowptr, 64)
+    return (structure2(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind, th),
+            O.scan2(m.rowptr, th, 64, 32))
+
+
+def decode_node_columns(eng, L):
+    """The column each slot reads through the node window, from the engine's
+    arrays: entry index from the lane's word, then the entry's three columns."""
+    off = eng.download(capi.WK_ENGINE_WIN_OFF).astype(np.int64)
+    nc = eng.download(capi.WK_ENGINE_ND_COLS).astype(np.int64)
+    cnt = eng.download(capi.WK_ENGINE_ND_CNT).astype(np.int64)
+    noff = eng.download(capi.WK_ENGINE_ND_OFF).astype(np.int64)
+    pos = eng.download(capi.WK_ENGINE_ND_POS)
+    G = L.nwarps
+    lens = L.max_nz_rows.astype(np.int64)
+    assert cnt.size == G and noff.size == G + 1 and nc.size == off[-1]
+    assert np.array_equal(np.diff(noff), 3 * ((lens + 26) // 27)) and pos.size == noff[-1] * 64
+    assert np.all(cnt <= np.diff(off)) and np.all(cnt <= 1024)
+    # per slot (group, step, lane), in slot order
+    group = np.repeat(np.arange(G), lens * 64)
+    within = np.arange(group.size, dtype=np.int64) - L.warp_offsets[group]
+    step, lane = within // 64, within % 64
+    trip = step // 3
+    word = pos[(noff[group] + trip // 3) * 64 + lane].astype(np.int64)
+    ent = (word >> (10 * (trip % 3))) & 1023
+    assert np.all(ent < cnt[group])
+    e = nc[off[group] + ent]
+    return np.where(e >= 0, e + step % 3, -e - 1), cnt
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_node_windows_name_each_slots_column(variant):
+    say(f"[gpu] node windows decode, variant {variant}")
+    m = matrix("hex_elastic_3d_32")
+    s, L = layout(m, variant)
+    eng = engine(s, m.vals, m.colind)
+    cols, cnt = decode_node_columns(eng, L)
+    _, pc = O.process(L, np.zeros(m.nnz), m.colind)  # pads read column 0
+    assert np.array_equal(cols, pc)
+    # one entry per node a group touches (a third of its window columns), and
+    # one for pads where the group has them
+    wsz = np.diff(eng.download(capi.WK_ENGINE_WIN_OFF).astype(np.int64))
+    assert np.all(cnt * 3 >= wsz) and np.all(cnt * 3 <= wsz + 3)
+    eng.close()
+
+
+@pytest.mark.parametrize("name", ["hex_laplace_3d_48", "tet_poisson_3d_96", "q1_laplace_2d_128"])
+def test_scalar_layouts_have_no_node_windows(name):
+    """Scalar problems: rows are not whole nodes, so stage 7 runs as stage 5."""
+    say(f"[gpu] no node windows on {name}")
+    m = matrix(name)
+    s, _ = layout(m, 1)
+    eng = engine(s, m.vals, m.colind)
+    assert eng.download(capi.WK_ENGINE_ND_CNT).size == 0
+    x = dev(fe.uniform_x(m.ncols, seed=7101))
+    y5, y7 = nan_like(m.nrows), nan_like(m.nrows)
+    eng.launch(x, y5, cfg=PK)
+    eng.launch(x, y7, cfg=ND)
+    assert torch.equal(y5, y7)
+    eng.close()
+
+
+@pytest.mark.parametrize("name", ["hex_elastic_3d_32", "hex_elastic_3d_64"])
+@pytest.mark.parametrize("variant", [1, 2])
+def test_node_kernel_bitexact(name, variant):
+    say(f"[gpu] node kernel {name} variant {variant}")
+    m = matrix(name)
+    s, L = layout(m, variant)
+    eng = engine(s, m.vals, m.colind)
+    assert eng.download(capi.WK_ENGINE_ND_CNT).size == L.nwarps
+    x = fe.uniform_x(m.ncols, seed=7102)
+    if variant == 1:
+        ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    else:
+        pv, pc = O.process(L, m.vals, m.colind)
+        ref = O.wpk_spmv(L, pv, pc, x)
+    b = fe.uniform_x(m.nrows, seed=7103)
+    # unroll 16: 27-step batches, 8: 18-step
+    for wpb, remap, u in ((1, 1, 16), (2, 0, 8), (4, 1, 16), (1, 1, 8)):
+        say(f"[gpu]   waves_per_block {wpb} xcd_remap {remap} unroll {u}")
+        cfg = dict(ND, waves_per_block=wpb, xcd_remap=remap, unroll=u)
+        y = nan_like(m.nrows)
+        eng.launch(dev(x), y, cfg=cfg)
+        assert np.array_equal(host(y), ref)
+        # the residual through the same kernel
+        r, r5 = nan_like(m.nrows), nan_like(m.nrows)
+        eng.residual(dev(b), dev(x), r, cfg=cfg)
+        eng.residual(dev(b), dev(x), r5, cfg=dict(cfg, stage_lds=5))
+        assert torch.equal(r, r5) and np.array_equal(host(r), b - ref)
+    eng.close()
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_node_kernel_strided_columns(k):
+    """k right-hand sides at stage 7: one launch per column of x (row stride
+    k), each equal to the single-column product."""
+    say(f"[gpu] node kernel k={k}")
+    m = matrix("hex_elastic_3d_32")
+    s, _ = layout(m, 1)
+    eng = engine(s, m.vals, m.colind)
+    X = fe.uniform_x(m.ncols, k, seed=7104)
+    Y = nan_like(m.nrows, k)
+    eng.launch(dev(X), Y, cfg=dict(ND, unroll=8 if k == 3 else 16))
+    Yh = host(Y)
+    for j in range(k):
+        assert np.array_equal(Yh[:, j], O.csr_spmv(m.rowptr, m.colind, m.vals, X[:, j]))
+    eng.close()
+
+
+@pytest.mark.parametrize("acc", [0, 1])
+def test_node_kernel_fp32_matches_packed(acc):
+    say(f"[gpu] node kernel fp32 accumulate {acc}")
+    m = matrix("hex_elastic_3d_32")
+    s, _ = layout(m, 1)
+    eng = engine(s, m.vals, m.colind, dtype="f32")
+    assert eng.download(capi.WK_ENGINE_ND_CNT).size > 0
+    x = dev(fe.uniform_x(m.ncols, seed=7105).astype(np.float32))
+    y5, y7 = nan_like(m.nrows), nan_like(m.nrows)
+    for wpb, u in ((1, 16), (4, 8), (2, 8)):
+        eng.launch(x, y5, cfg=dict(PK, waves_per_block=wpb, accumulate=acc, unroll=u))
+        eng.launch(x, y7, cfg=dict(ND, waves_per_block=wpb, accumulate=acc, unroll=u))
+        assert torch.equal(y5, y7)
+    eng.close()
+
+
+def test_node_windows_from_the_device_scan_and_run_x():
+    """Engines scanned on the device get the same node windows; run_x (Kr
+    numbering) runs as stage 5."""
+    say("[gpu] node windows, device scan and run_x")
+    m = matrix("hex_elastic_3d_32")
+    s, L = layout(m, 1)
+    he = engine(s, m.vals, m.colind, kr=True)
+    de = engine.from_device(dev(m.rowptr), dev(m.colind), dev(m.vals), ncols=m.ncols, kr=True)
+    for w in (capi.WK_ENGINE_ND_COLS, capi.WK_ENGINE_ND_CNT, capi.WK_ENGINE_ND_OFF, capi.WK_ENGINE_ND_POS):
+        a, b = he.download(w), de.download(w)
+        if w == capi.WK_ENGINE_ND_COLS:  # only each group's first ND_CNT entries are written
+            off = he.download(capi.WK_ENGINE_WIN_OFF).astype(np.int64)
+            cnt = he.download(capi.WK_ENGINE_ND_CNT).astype(np.int64)
+            keep = np.zeros(a.size, bool)
+            for g in range(L.nwarps):
+                keep[off[g]:off[g] + cnt[g]] = True
+            a, b = a[keep], b[keep]
+        assert np.array_equal(a, b), w
+    x = fe.uniform_x(m.ncols, seed=7106)
+    ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    y = nan_like(m.nrows)
+    de.launch(dev(x), y, cfg=ND)
+    assert np.array_equal(host(y), ref)
+    yx = nan_like(m.nrows)
+    he.launch_x(dev(s.reorder_x(x)), yx, cfg=ND)
+    assert np.array_equal(host(yx), ref[s.row_map])
+    he.close()
+    de.close()

@@ -155,7 +155,7 @@ def test_swapped_columns_drop_and_restore_the_windows(where):
     put = (lambda a: a) if where == "host" else dev
     restore = eng.device_colinds
     reorder_cols = s.reorder_columns_coalesced(m.colind)
-    for st in (3, 5):
+    for st in (3, 5, 7):
         eng.device_colinds = put(reorder_cols)
         assert eng.x_window_entries() == 0
         yx = nan_vec(m.nrows)

@@ -256,7 +256,8 @@ class engine:
 
     def download(self, which: int) -> np.ndarray:
         """One device array of the forward layout (WK_ROW_MAP .. WK_ROW_OFFSET_WARP,
-        WK_ENGINE_VALUES / COLS / COLS_KR); WK_REORDER_ROWS is the CSR-to-slot map."""
+        WK_ENGINE_VALUES / COLS / COLS_KR, the x windows WK_ENGINE_WIN_* and the
+        node windows WK_ENGINE_ND_*); WK_REORDER_ROWS is the CSR-to-slot map."""
         dt = {capi.WK_ROW_MAP: np.int32, capi.WK_ROW_MAP_INV: np.int32,
               capi.WK_REORDER_ROWS: np.uint32, capi.WK_WARP_OFFSETS: np.int64,
               capi.WK_MAX_NZ_ROWS: np.uint32, capi.WK_THREADS_PER_ROW: np.uint32,
@@ -264,7 +265,9 @@ class engine:
               capi.WK_ENGINE_VALUES: np.float64 if self.dtype == capi.WK_F64 else np.float32,
               capi.WK_ENGINE_COLS: np.int32, capi.WK_ENGINE_COLS_KR: np.int32,
               capi.WK_ENGINE_WIN_IDX: np.uint16, capi.WK_ENGINE_WIN_OFF: np.uint32,
-              capi.WK_ENGINE_WIN_COLS: np.int32, capi.WK_ENGINE_WIN_COLS_KR: np.int32}[which]
+              capi.WK_ENGINE_WIN_COLS: np.int32, capi.WK_ENGINE_WIN_COLS_KR: np.int32,
+              capi.WK_ENGINE_ND_COLS: np.int32, capi.WK_ENGINE_ND_CNT: np.uint32,
+              capi.WK_ENGINE_ND_OFF: np.uint32, capi.WK_ENGINE_ND_POS: np.uint32}[which]
         n = C.c_int64()
         check(lib().wk_engine_download(self._h, which, None, C.byref(n)))
         out = np.empty(n.value, dt)

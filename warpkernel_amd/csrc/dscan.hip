@@ -585,6 +585,17 @@ extern "C" int wk_engine_download(const wk_engine *E, int32_t which, void *host_
         n = src ? total : 0;
         break;
       }
+      case WK_ENGINE_ND_COLS: src = D.nd_cols; n = D.nd_cols ? D.win_total : 0; break;
+      case WK_ENGINE_ND_CNT: src = D.nd_cnt; n = D.nd_cnt ? D.ngroups : 0; break;
+      case WK_ENGINE_ND_OFF: src = D.nd_off; n = D.nd_off ? D.ngroups + 1 : 0; break;
+      case WK_ENGINE_ND_POS: {
+        src = D.nd_pos;
+        uint32_t words = 0;
+        if (src && D.nd_off)
+          hip_check(hipMemcpy(&words, D.nd_off + D.ngroups, sizeof words, hipMemcpyDeviceToHost), "download");
+        n = src ? (int64_t)words * 64 : 0;
+        break;
+      }
       default: fail(WK_ERR_ARG, "unknown engine array");
     }
     if (!count) fail(WK_ERR_ARG, "count is NULL");

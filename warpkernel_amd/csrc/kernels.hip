@@ -242,6 +242,10 @@ struct WinArgs {
   int32_t win_bytes;     // window part (16-byte multiple)
   const uint32_t *pk;      // packed positions (spmv_winpk_kernel)
   const uint32_t *pk_off;  // [ngroups+1] first 12-step block of each group
+  const int32_t *nd_cols;  // node entries from off[g] (spmv_winnd_kernel)
+  const uint32_t *nd_cnt;  // [ngroups]
+  const uint32_t *nd_pos;  // node positions, word-major
+  const uint32_t *nd_off;  // [ngroups+1] first word of each group
 };
 
 // Work block -> (group block, k-tile), k-tile fastest: the blocks of one
@@ -376,6 +380,64 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_winpk_kernel(SpmvArgs a, Win
   f.gather(xs, w.cols + w0, w1 - w0, static_cast<const X *>(a.x), a.ldx, lane);
   Acc acc;
   window_group_pk<P, NB>(acc, vbase, pk, xs, m.len, bt, early, xs + w.win_bytes, lane);
+  if constexpr (SPLIT) {
+    for (uint32_t o = 1; o < m.tpr; o <<= 1) {
+      Acc other = P::shfl_down(acc, (int)o, W);
+      P::add(acc, other);
+    }
+  }
+  const uint32_t rowid = m.row0 + (uint32_t)lane / m.tpr;
+  if (((uint32_t)lane & (m.tpr - 1)) != 0 || rowid >= (uint32_t)a.nznrows) return;
+  const int64_t dst = OUT == OUT_RUNX ? (int64_t)rowid : (int64_t)a.row_map[rowid];
+  const double res = P::result(acc);
+  if constexpr (OUT == OUT_RESID)
+    a.y[dst * a.ldy] = a.b[dst * a.ldb] - res;
+  else
+    a.y[dst * a.ldy] = res;
+}
+
+// One right-hand side through a node window (layouts whose slots read x in
+// aligned column triples, as elasticity's three unknowns per node): the
+// same results as spmv_winpk_kernel, with one 4-byte window entry per node
+// instead of three and 10 bits of position per three slots instead of per
+// slot.  Batches of 9 * NW steps of values through the slice, positions in
+// registers.
+template <class P, bool SPLIT, int OUT, int NW>
+__global__ __launch_bounds__(kMaxThreads) void spmv_winnd_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
+                                                                 int32_t remap) {
+  using V = typename P::V;
+  using X = typename P::X;
+  using Acc = typename P::Acc;
+  constexpr int W = 64;
+  constexpr int U = WinBatchNd<P, NW>::U;
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t wpb = blockDim.x >> 6;
+  const int32_t bid = blockIdx.x;
+  if (bid >= nwork) {
+    const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
+    if (pos >= a.nrows) return;
+    const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
+    a.y[dst * a.ldy] = OUT == OUT_RESID ? a.b[dst * a.ldb] : 0.0;
+    return;
+  }
+  const int32_t wib = threadIdx.x >> 6;
+  const int32_t wave = __builtin_amdgcn_readfirstlane(work_id(bid, nwork, remap) * wpb + wib);
+  if (wave >= a.ngroups) return;
+  const GroupMeta m = load_meta<W, SPLIT>(a, wave);
+  const V *vbase = static_cast<const V *>(a.vals) + m.base;
+  const uint32_t *pw = w.nd_pos + (int64_t)w.nd_off[wave] * 64 + lane;
+  char *xs = lds + wib * w.wave_bytes;
+  const int32_t *nc = w.nd_cols + w.off[wave];
+  const uint32_t nn = w.nd_cnt[wave];
+  NdFill<X> f;
+  f.cols(nc, nn, lane);
+  WinBatchNd<P, NW> bt;
+  const bool early = m.len >= (uint32_t)U;
+  if (early) bt.issue(vbase, pw, 0, lane);  // in flight while the window fills
+  f.gather(xs, nc, nn, static_cast<const X *>(a.x), a.ldx, lane);
+  Acc acc;
+  window_group_nd<P, NW>(acc, vbase, pw, xs, m.len, bt, early, xs + w.win_bytes, lane);
   if constexpr (SPLIT) {
     for (uint32_t o = 1; o < m.tpr; o <<= 1) {
       Acc other = P::shfl_down(acc, (int)o, W);
@@ -997,6 +1059,40 @@ WinPkFn pick_winpk(int prec, bool split, int out, int nb) {
   return nb == 1 ? pick_winpk_split<PF32D, 1>(split, out) : pick_winpk_split<PF32D, 2>(split, out);
 }
 
+template <class P, bool SPLIT, int OUT, int NW>
+void launch_winnd(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
+                  int32_t nwork, int32_t remap, hipStream_t s) {
+  hipLaunchKernelGGL((spmv_winnd_kernel<P, SPLIT, OUT, NW>), grid, block, lds, s, a, w, nwork, remap);
+}
+
+// node windows read x in the product's own numbering: run and the residual;
+// nw: words (9 steps each) per batch, 2 or 3
+template <class P, bool SPLIT, int NW>
+WinPkFn pick_winnd_out(int out) {
+  if (out == OUT_RUN) return &launch_winnd<P, SPLIT, OUT_RUN, NW>;
+  if constexpr (std::is_same_v<P, PF64>)
+    if (out == OUT_RESID) return &launch_winnd<P, SPLIT, OUT_RESID, NW>;
+  return nullptr;
+}
+
+template <class P, int NW>
+WinPkFn pick_winnd_split(bool split, int out) {
+  return split ? pick_winnd_out<P, true, NW>(out) : pick_winnd_out<P, false, NW>(out);
+}
+
+template <class P>
+WinPkFn pick_winnd_nw(bool split, int out, int nw) {
+  if (nw == 2) return pick_winnd_split<P, 2>(split, out);
+  if (nw == 3) return pick_winnd_split<P, 3>(split, out);
+  return nullptr;
+}
+
+WinPkFn pick_winnd(int prec, bool split, int out, int nw) {
+  if (prec == 0) return pick_winnd_nw<PF64>(split, out, nw);
+  if (prec == 1) return pick_winnd_nw<PF32C>(split, out, nw);
+  return pick_winnd_nw<PF32D>(split, out, nw);
+}
+
 using WinResFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
                           int32_t, hipStream_t);
 
@@ -1045,7 +1141,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
                  int64_t ldx, double *y, int64_t ldy, const double *b, int64_t ldb, int32_t k,
                  hipStream_t stream) {
   if (k < 1) fail(WK_ERR_ARG, "k must be >= 1");
-  if (cfg.stage_lds < 0 || cfg.stage_lds > 6) fail(WK_ERR_ARG, "stage_lds must be 0 to 6");
+  if (cfg.stage_lds < 0 || cfg.stage_lds > 7) fail(WK_ERR_ARG, "stage_lds must be 0 to 7");
   if (cfg.nontemporal < 0 || cfg.nontemporal > 2) fail(WK_ERR_ARG, "nontemporal must be 0, 1 or 2");
   const int wpb = cfg.waves_per_block;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) fail(WK_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
@@ -1128,7 +1224,42 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const bool windowed = cfg.stage_lds >= 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
                           cfg.nontemporal &&
                           (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
-    if (windowed && cfg.stage_lds >= 5 && kt == 1 && D.win_pk) {  // 5, and the rest of 6
+    if (windowed && cfg.stage_lds == 7 && kt == 1 && D.nd_max > 0) {
+      // node windows: per wave the window of node entries, then a slice of
+      // values (unroll 16: 27-step batches, 8: 18-step)
+      const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
+      const int nw = cfg.unroll == 16 ? 3 : 2;
+      const size_t win_bytes = ((size_t)D.nd_max * 3 * xsz + 15) / 16 * 16;
+      const size_t wave_bytes = win_bytes + (size_t)9 * nw * 64 * vbytes;
+      const size_t need = wave_bytes * (size_t)wpb;
+      WinPkFn nf = pick_winnd(prec, split, out, nw);
+      if (nf && need <= (size_t)lds_limits().per_block) {
+        WinArgs w{};
+        w.off = D.win_off;
+        w.wave_bytes = (int32_t)wave_bytes;
+        w.win_bytes = (int32_t)win_bytes;
+        w.nd_cols = D.nd_cols;
+        w.nd_cnt = D.nd_cnt;
+        w.nd_pos = D.nd_pos;
+        w.nd_off = D.nd_off;
+        const int64_t nwork = ((int64_t)D.ngroups + wpb - 1) / wpb;
+        const int64_t nfill = ((int64_t)D.nrows - D.nznrows + wpb * 64 - 1) / (wpb * 64);
+        if (nwork + nfill > INT32_MAX) fail(WK_ERR_LIMIT, "grid too large");
+        for (int32_t t = 0; t < ntiles; t++) {
+          a.x = static_cast<const char *>(x) + (size_t)(j0 + t) * xsz;
+          a.y = y + j0 + t;
+          a.b = b ? b + j0 + t : nullptr;
+          a.k = 1;
+          if (nwork + nfill > 0)
+            nf(a, w, dim3((uint32_t)(nwork + nfill)), dim3(wpb * 64), std::max(need, lds),
+               (int32_t)nwork, cfg.xcd_remap ? 1 : 0, stream);
+          hip_check(hipGetLastError(), "spmv launch");
+        }
+        j0 += ntiles;
+        continue;
+      }
+    }
+    if (windowed && cfg.stage_lds >= 5 && kt == 1 && D.win_pk) {  // 5, the rest of 6, 7 elsewhere
       // packed positions: per wave the window, then a 12-step slice of values
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
       const size_t win_bytes = ((size_t)D.win_max * xsz + 15) / 16 * 16;
@@ -1327,6 +1458,143 @@ __global__ __launch_bounds__(256) void pack_positions_kernel(const uint16_t *__r
   }
 }
 
+// Node windows from the windows of a layout, one wave per group.  Pass 1
+// (nd_pos null) checks that every triple of a lane's steps reads either a node
+// (three consecutive window positions holding columns 3n, 3n+1, 3n+2) or one
+// position three times, and counts the group's entries: per window position
+// in order, a node entry where a node starts there, then a constant entry
+// where some triple reads that position three times.  It writes the count, or
+// kNdNone when the group does not fit.  Pass 2 (every group fits) writes the
+// entries and each lane's entry indices, three 10-bit fields to a word.
+constexpr uint32_t kNdNone = 0xffffffffu;
+constexpr int kNdWaves = 4;
+
+__global__ __launch_bounds__(kNdWaves * 64) void node_window_kernel(
+    const uint16_t *__restrict__ idx, const int64_t *__restrict__ warp_offsets,
+    const uint32_t *__restrict__ max_nz_rows, const uint32_t *__restrict__ win_off,
+    const int32_t *__restrict__ win_cols, int32_t ngroups, uint32_t *__restrict__ cnt,
+    const uint32_t *__restrict__ nd_off, int32_t *__restrict__ nd_cols,
+    uint32_t *__restrict__ nd_pos) {
+  // per window position: bit 0 a node starts here, bit 1 read as a constant triple
+  __shared__ uint8_t kind[kNdWaves][kPackedWindow];
+  // per window position: the entry of the node starting there, of the constant
+  __shared__ uint16_t entry_of[kNdWaves][kPackedWindow][2];
+  const int32_t wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int32_t g = (int32_t)blockIdx.x * kNdWaves + wib;
+  const bool live = g < ngroups;
+  uint32_t n = 0, len = 0, w0 = 0;
+  int64_t base = 0;
+  if (live) {
+    w0 = win_off[g];
+    n = win_off[g + 1] - w0;  // at most kPackedWindow (packed windows only)
+    len = max_nz_rows[g];
+    base = warp_offsets[g];
+  }
+  uint8_t *kd = kind[wib];
+  const int32_t *wc = win_cols + w0;
+  for (uint32_t p = (uint32_t)lane; p < n; p += 64) {
+    const int32_t c = wc[p];
+    const bool st = c % 3 == 0 && p + 2 < n && wc[p + 1] == c + 1 && wc[p + 2] == c + 2;
+    kd[p] = st ? 1 : 0;
+  }
+  __syncthreads();
+  bool bad = live && len % 3 != 0;
+  if (live && len % 3 == 0) {
+    for (uint32_t t = 0; t < len; t += 3) {
+      const int64_t s = base + lane + 64 * (int64_t)t;
+      const uint32_t p0 = idx[s], p1 = idx[s + 64], p2 = idx[s + 128];
+      if (p1 == p0 + 1 && p2 == p0 + 2 && (kd[p0] & 1)) continue;
+      if (p0 == p1 && p1 == p2)
+        kd[p0] |= 2;  // lanes only set bit 1 here: concurrent writes agree
+      else
+        bad = true;
+    }
+  }
+  __syncthreads();
+  uint32_t total = 0;
+  const uint64_t below = (1ull << lane) - 1;
+  for (uint32_t p0 = 0; p0 < n; p0 += 64) {
+    const uint32_t p = p0 + (uint32_t)lane;
+    const uint32_t k = p < n ? kd[p] : 0u;
+    const uint64_t ms = __ballot(k & 1u), mc = __ballot((k >> 1) & 1u);
+    const uint32_t r = total + (uint32_t)__popcll(ms & below) + (uint32_t)__popcll(mc & below);
+    if (k & 1u) {
+      entry_of[wib][p][0] = (uint16_t)r;
+      if (nd_pos) nd_cols[w0 + r] = wc[p];
+    }
+    if (k & 2u) {
+      const uint32_t rc = r + (k & 1u);
+      entry_of[wib][p][1] = (uint16_t)rc;
+      if (nd_pos) nd_cols[w0 + rc] = -wc[p] - 1;
+    }
+    total += (uint32_t)__popcll(ms) + (uint32_t)__popcll(mc);
+  }
+  bad = __ballot(bad) != 0 || total > n;
+  __syncthreads();
+  if (!live) return;
+  if (!nd_pos) {
+    if (lane == 0) cnt[g] = bad ? kNdNone : total;
+    return;
+  }
+  // pass 2: each lane's entry index per triple of steps, 27 steps to 3 words
+  const uint32_t nblk = (len + 26) / 27;
+  uint32_t *dst = nd_pos + (int64_t)nd_off[g] * 64 + lane;
+  for (uint32_t bk = 0; bk < nblk; bk++)
+    for (uint32_t q = 0; q < 3; q++) {
+      uint32_t word = 0;
+      for (uint32_t f = 0; f < 3; f++) {
+        const uint32_t t = bk * 27 + q * 9 + f * 3;
+        if (t < len) {
+          const int64_t s = base + lane + 64 * (int64_t)t;
+          const uint32_t p0 = idx[s], p1 = idx[s + 64];
+          word |= (uint32_t)entry_of[wib][p0][p1 == p0 + 1 ? 0 : 1] << (10 * f);
+        }
+      }
+      dst[((int64_t)bk * 3 + q) * 64] = word;
+    }
+}
+
+static void build_node_windows(wk_engine *E, DevLayout &D, hipStream_t stream) {
+  D.nd_cols = nullptr;
+  D.nd_cnt = nullptr;
+  D.nd_pos = nullptr;
+  D.nd_off = nullptr;
+  D.nd_max = 0;
+  const int32_t G = D.ngroups;
+  if (!D.win_pk || !D.win_cols || !D.win_off || D.win_total <= 0 || G <= 0) return;
+  for (int32_t g = 0; g < G; g++)
+    if (D.h_max_nz_rows[g] % 3) return;
+  const int64_t blocks = ((int64_t)G + kNdWaves - 1) / kNdWaves;
+  uint32_t *cnt = dev_alloc<uint32_t>(E, G);
+  hipLaunchKernelGGL(node_window_kernel, dim3((uint32_t)blocks), dim3(kNdWaves * 64), 0, stream,
+                     D.win_idx, D.warp_offsets, D.max_nz_rows, D.win_off, D.win_cols, G, cnt,
+                     nullptr, nullptr, nullptr);
+  hip_check(hipGetLastError(), "node window launch");
+  std::vector<uint32_t> h((size_t)G);
+  hip_check(hipMemcpyAsync(h.data(), cnt, sizeof(uint32_t) * (size_t)G, hipMemcpyDeviceToHost, stream),
+            "node windows");
+  hip_check(hipStreamSynchronize(stream), "node windows");
+  uint32_t most = 0;
+  std::vector<uint32_t> off((size_t)G + 1, 0);
+  for (int32_t g = 0; g < G; g++) {
+    if (h[g] == kNdNone) return;
+    most = std::max(most, h[g]);
+    const uint64_t words = (uint64_t)off[g] + 3ull * ((D.h_max_nz_rows[g] + 26) / 27);
+    if (words > UINT32_MAX) return;
+    off[g + 1] = (uint32_t)words;
+  }
+  D.nd_off = dev_upload(E, off.data(), (int64_t)off.size(), stream);
+  D.nd_pos = dev_alloc<uint32_t>(E, (int64_t)off.back() * 64);
+  D.nd_cols = dev_alloc<int32_t>(E, D.win_total);
+  hipLaunchKernelGGL(node_window_kernel, dim3((uint32_t)blocks), dim3(kNdWaves * 64), 0, stream,
+                     D.win_idx, D.warp_offsets, D.max_nz_rows, D.win_off, D.win_cols, G, cnt,
+                     D.nd_off, D.nd_cols, D.nd_pos);
+  hip_check(hipGetLastError(), "node window launch");
+  hip_check(hipStreamSynchronize(stream), "node windows");
+  D.nd_cnt = cnt;
+  D.nd_max = (int32_t)most;
+}
+
 void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream) {
   D.win_pk = nullptr;
   D.win_pk_off = nullptr;
@@ -1347,6 +1615,7 @@ void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream) {
                        reinterpret_cast<u32x4 *>(D.win_pk));
   hip_check(hipGetLastError(), "pack launch");
   hip_check(hipStreamSynchronize(stream), "pack");
+  build_node_windows(E, D, stream);
 }
 
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream) {

@@ -626,6 +626,160 @@ __device__ __forceinline__ void window_group_pk(typename P::Acc &acc, const type
   }
 }
 
+// ------------------------------------------------------- node windows
+// One right-hand side through a window of node entries (three x values
+// each).  A lane's steps go three at a time: steps 3m, 3m+1, 3m+2 read values
+// 0, 1, 2 of one entry, whose 10-bit index is field m % 3 of word m / 3 of
+// the lane's positions.  A batch of NW words covers 9 * NW steps: at 27
+// steps, three words per lane against nine of 16-bit positions or 2.25
+// 16-byte blocks of 10-bit ones.
+
+// Batch of 9 * NW steps: the values as 16-byte pieces of the group's
+// column-major slots (the last 1 KB chunk may be partial) and NW words.
+template <class P, int NW>
+struct WinBatchNd {
+  static constexpr int U = 9 * NW;
+  static constexpr int kPieces = U * 64 * (int)sizeof(typename P::V) / 16;  // per batch
+  static constexpr int NV = (kPieces + 63) / 64;
+  static_assert(kPieces * 16 == U * 64 * (int)sizeof(typename P::V), "whole 16-byte pieces");
+  u32x4 tv[NV];
+  uint32_t tp[NW];
+  // pw: the group's words plus the lane (word q of the lane at pw[q * 64])
+  __device__ __forceinline__ void issue(const typename P::V *vbase, const uint32_t *pw, uint32_t b,
+                                        int32_t lane) {
+#pragma unroll
+    for (int q = 0; q < NW; q++) tp[q] = pw[((int64_t)b * NW + q) * 64];
+    const u32x4 *gv = reinterpret_cast<const u32x4 *>(vbase + (int64_t)b * U * 64);
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+      if (q < NV - 1 || q * 64 + lane < kPieces) tv[q] = __builtin_nontemporal_load(gv + q * 64 + lane);
+  }
+  __device__ __forceinline__ void stage(char *slice, int32_t lane) const {
+    u32x4 *lv = reinterpret_cast<u32x4 *>(slice);
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+      if (q < NV - 1 || q * 64 + lane < kPieces) lv[q * 64 + lane] = tv[q];
+  }
+  // byte offset in the window of the entry that steps 3m .. 3m+2 read
+  __device__ static __forceinline__ uint32_t entry(const uint32_t (&p)[NW], int m) {
+    return ((p[m / 3] >> (10 * (m % 3))) & 1023u) * 3u * (uint32_t)sizeof(typename P::X);
+  }
+};
+
+// The steps of one staged batch: value from the slice, x from the window.
+template <class P, int NW>
+__device__ __forceinline__ void nd_batch(typename P::Acc &acc, const typename P::V *lv,
+                                         const char *xs, const uint32_t (&p)[NW], int32_t lane) {
+  using X = typename P::X;
+  constexpr int U = 9 * NW;
+  X xv[U];
+#pragma unroll
+  for (int m = 0; m < U / 3; m++) {
+    const X *e = reinterpret_cast<const X *>(xs + WinBatchNd<P, NW>::entry(p, m));
+#pragma unroll
+    for (int d = 0; d < 3; d++) xv[3 * m + d] = e[d];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
+}
+
+// Fills a node window: entry j < n (the group's entries nc[0, n)) gets its
+// three x values.  Four entries per lane in flight, twelve loads.
+template <class X>
+struct NdFill {
+  int32_t e[4];
+  __device__ __forceinline__ void cols(const int32_t *__restrict__ nc, uint32_t n, int32_t lane) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t j = (uint32_t)lane + q * 64;
+      e[q] = j < n ? nc[j] : -1;
+    }
+  }
+  __device__ __forceinline__ void gather(char *xs, const int32_t *__restrict__ nc, uint32_t n,
+                                         const X *__restrict__ xb, int64_t ldx, int32_t lane) {
+    for (uint32_t j0 = 0;;) {
+      X v[4][3];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        // e >= 0: columns e, e+1, e+2; e < 0: column -e-1 three times
+        const int64_t c = e[q] >= 0 ? e[q] : -(int64_t)e[q] - 1;
+        const int64_t s = e[q] >= 0 ? 1 : 0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[q][d] = xb[(c + d * s) * ldx];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t j = j0 + (uint32_t)lane + q * 64;
+        if (j < n) {
+          X *dst = reinterpret_cast<X *>(xs + j * 3u * (uint32_t)sizeof(X));
+#pragma unroll
+          for (int d = 0; d < 3; d++) dst[d] = v[q][d];
+        }
+      }
+      j0 += 256;
+      if (j0 >= n) break;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t j = j0 + (uint32_t)lane + q * 64;
+        e[q] = j < n ? nc[j] : -1;
+      }
+    }
+  }
+};
+
+// One group through its node window: the same products in the same order as
+// window_group_pk (every slot by fma onto -0, pads included).  len is a
+// multiple of three.  When `issued`, the caller has issued batch 0 into bt
+// before filling the window.
+template <class P, int NW>
+__device__ __forceinline__ void window_group_nd(typename P::Acc &acc, const typename P::V *vbase,
+                                                const uint32_t *pw, const char *xs, uint32_t len,
+                                                WinBatchNd<P, NW> &bt, bool issued, char *slice,
+                                                int32_t lane) {
+  using V = typename P::V;
+  using X = typename P::X;
+  constexpr int U = WinBatchNd<P, NW>::U;
+  const V *lv = reinterpret_cast<const V *>(slice);
+  P::init(acc);
+  uint32_t i = 0, b = 0;
+  if (i + U <= len) {
+    if (!issued) bt.issue(vbase, pw, 0, lane);
+    for (; i + 2 * U <= len; i += U, b++) {
+      bt.stage(slice, lane);
+      uint32_t p[NW];
+#pragma unroll
+      for (int q = 0; q < NW; q++) p[q] = bt.tp[q];
+      bt.issue(vbase, pw, b + 1, lane);
+      nd_batch<P, NW>(acc, lv, xs, p, lane);
+    }
+    bt.stage(slice, lane);
+    uint32_t p[NW];
+#pragma unroll
+    for (int q = 0; q < NW; q++) p[q] = bt.tp[q];
+    nd_batch<P, NW>(acc, lv, xs, p, lane);
+    i += U;
+    b++;
+  }
+  // the last, partial batch (fewer than U steps, a multiple of three): the
+  // words it has and direct value loads
+  if (i < len) {
+    uint32_t p[NW];
+#pragma unroll
+    for (int q = 0; q < NW; q++) p[q] = i + 9 * q < len ? pw[((int64_t)b * NW + q) * 64] : 0u;
+    V v[U - 3];
+#pragma unroll
+    for (int u = 0; u < U - 3; u++)
+      if (i + u < len) v[u] = __builtin_nontemporal_load(vbase + (int64_t)(i + u) * 64 + lane);
+#pragma unroll
+    for (int m = 0; m < U / 3 - 1; m++)
+      if (i + 3 * m < len) {
+        const X *e = reinterpret_cast<const X *>(xs + WinBatchNd<P, NW>::entry(p, m));
+#pragma unroll
+        for (int d = 0; d < 3; d++) P::step(acc, v[3 * m + d], e[d]);
+      }
+  }
+}
+
 // The x reads and FMAs of one staged batch: slot u of the lane reads its
 // value and window position from the slice, then its KW x values (values
 // t0 .. t0+KW-1 of the entry).

@@ -64,6 +64,10 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
                 c.stage_lds = 5;
                 out.push_back(c);
               }
+              if (D.nd_max > 0) {
+                c.stage_lds = 7;
+                out.push_back(c);
+              }
             }
             c.stage_lds = 0;
           }
@@ -159,9 +163,20 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     // slot (stage 5: 24- or 12-step batches; 64^3: 96.7 us, 32^3: 13.8
     // against 15.0, WPK2 64^3: 107.5 against 110.2).  FP32 values take the
     // 12-step batches at every length (32^3 compensated 9.3 us against 9.6,
-    // FP64 accumulation 8.8 against 12.1; 64^3 58.4 / 58.5, 58.2 / 71.4)
+    // FP64 accumulation 8.8 against 12.1; 64^3 58.4 / 58.5, 58.2 / 71.4).
+    // Where slots read x by whole nodes (elasticity) the node window takes
+    // 10 bits of position per three slots (stage 7), with 27-step batches
+    // on long groups and 18-step ones otherwise: 64^3 90.0 against 97.3 us,
+    // 32^3 12.6 against 14.4, WPK2 at threshold 27 106.3 against 108.6 and
+    // 16.0 against 17.5; FP32 64^3 51.4 against 58.6 (compensated) and 50.4
+    // against 58.9 (FP64 accumulation), 32^3 7.8 against 9.7 and 7.3 against
+    // 9.3
     c.stage_lds = D.win_pk ? 5 : 3;
     c.unroll = mean_len >= 40.0 && dtype == WK_F64 ? 16 : 8;
+    if (D.nd_max > 0) {
+      c.stage_lds = 7;
+      c.unroll = mean_len >= 40.0 ? 16 : 8;
+    }
     c.waves_per_block = 1;
     c.xcd_remap = 1;
   } else if (k >= 2 && windows) {
@@ -419,16 +434,31 @@ int wk_tune(wk_engine *E, int32_t op, const wk_launch *base, const void *x_dev, 
                        return p.first < q.first;
                      });
     const size_t top = std::min<size_t>(3, timed.size());
-    for (int pass = 0; pass < 2 && rc == WK_OK; pass++)
-      for (size_t i = 0; i < top && rc == WK_OK; i++) {
-        float ms = 0;
-        if ((rc = time_cfg(timed[i].second, ms)) == WK_OK) timed[i].first = std::min(timed[i].first, ms);
+    // the incumbent (base, or the engine's launch: the layout default until
+    // tuned) is timed beside them and kept unless one beats it by more than
+    // 1%, so shapes within noise of it do not replace it by chance
+    float inc_t = INFINITY;
+    for (int pass = 0; pass < 3 && rc == WK_OK; pass++) {
+      float ms = 0;
+      rc = time_cfg(b0, ms);
+      if (rc == WK_ERR_ARG) {
+        rc = WK_OK;  // not a shape this layout offers: no incumbent
+        break;
       }
+      if (rc == WK_OK) inc_t = std::min(inc_t, ms);
+      if (pass == 2) break;
+      for (size_t i = 0; i < top && rc == WK_OK; i++)
+        if ((rc = time_cfg(timed[i].second, ms)) == WK_OK) timed[i].first = std::min(timed[i].first, ms);
+    }
     for (size_t i = 0; i < top; i++)
       if (timed[i].first < best_t) {
         best_t = timed[i].first;
         best_c = timed[i].second;
       }
+    if (std::isfinite(inc_t) && !(best_t < 0.99f * inc_t)) {
+      best_t = inc_t;
+      best_c = b0;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

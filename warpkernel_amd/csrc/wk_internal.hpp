@@ -106,6 +106,19 @@ struct DevLayout {
   uint32_t *win_pk = nullptr;      // [win_pk_off[ngroups] * 64 * 4]
   uint32_t *win_pk_off = nullptr;  // [ngroups+1] first 12-step block of each group
   uint32_t pk_longest = 0;         // the longest group (steps), with win_pk
+  // node windows (stage_lds 7), built from the packed windows when every
+  // group's length is a multiple of three and each lane reads its window
+  // three steps at a time either at three consecutive positions holding
+  // columns 3n, 3n+1, 3n+2 (an elasticity node) or three times at one
+  // position (pads).  A group's window is then a list of node entries, each
+  // three x values in LDS: entry e >= 0 holds x[e], x[e+1], x[e+2]; e < 0
+  // holds x[-e-1] three times.  Per lane, one 10-bit entry index per three
+  // steps, three to a 32-bit word, three words per 27-step block.
+  int32_t *nd_cols = nullptr;      // [win_total] group g's entries from win_off[g]
+  uint32_t *nd_cnt = nullptr;      // [ngroups] entries per group
+  uint32_t *nd_pos = nullptr;      // [nd_off[ngroups] * 64] words, word-major, lane fastest
+  uint32_t *nd_off = nullptr;      // [ngroups+1] first word of each group
+  int32_t nd_max = 0;              // most entries in one group; 0: no node windows
 };
 
 // Window positions fit the packed form up to this many entries per window.
@@ -182,6 +195,7 @@ void launch_scatter(const void *src, const uint32_t *index, void *dst, int64_t n
                     int32_t elem_bytes, hipStream_t stream);
 // Builds D.win_pk / D.win_pk_off from D.win_idx when the windows fit the
 // packed form (width 64, win_max <= kPackedWindow); otherwise leaves them null.
+// Then builds the node windows (D.nd_*) where the layout has them.
 void build_packed_positions(wk_engine *E, DevLayout &D, hipStream_t stream);
 void launch_fill_zero(void *p, int64_t bytes, hipStream_t stream);
 // Flags (int32) the column swap's check kernels write, one per block.
