@@ -9,7 +9,7 @@ import pytest
 import bench
 
 LAUNCH = {"waves_per_block": 1, "unroll": 16, "k_tile": 1, "xcd_remap": 1, "nontemporal": 1,
-          "accumulate": 0, "rows_per_lane": 1, "blocks_per_cu": 0, "stage_lds": 5}
+          "accumulate": 0, "rows_per_lane": 1, "blocks_per_cu": 0, "stage_lds": 7}
 
 
 def _counter_dir(path, counter, kernel_values, copy_values):
@@ -61,4 +61,4 @@ def test_committed_headline_traffic_matches_the_sources():
     this tree with the headline's launch shape, so bench.py reports it."""
     got, src = bench.read_traffic("hex_elastic_3d_64", LAUNCH)
     assert got is not None, src
-    assert 5e8 < got < 8e8  # bytes per launch: about 661 MB, below the algorithmic 790 MB
+    assert 5e8 < got < 8e8  # bytes per launch: about 580 MB, below the algorithmic 790 MB
