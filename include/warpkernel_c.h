@@ -280,6 +280,9 @@ int wk_engine_create_device(int32_t variant, int32_t threshold, int32_t max_tpr,
 #define WK_ENGINE_ND_POS 26   /* uint32 [ND_OFF[nwarps] * 64] word w of lane l of group g at
                                  (ND_OFF[g] + w) * 64 + l: the 10-bit entries of steps
                                  9w+3f .. 9w+3f+2 in bits 10f .. 10f+9 */
+#define WK_ENGINE_ND_COLS_KR 27 /* int32 [3 * WIN_OFF[nwarps]] the entries in Kr numbering
+                                   (WK_ENGINE_KR engines): three columns each, entry j of
+                                   group g at 3 * (WIN_OFF[g] + j) */
 
 /* Copies one of the engine's device arrays of the forward layout to the host.
  * host_dst NULL: *count receives the array's length (0 for an array the engine
@@ -336,11 +339,13 @@ typedef struct wk_launch {
                               many values per column; the rest of k (fewer
                               than one tile), and engines it does not fit,
                               run as 4; 7 (k_tile 1, engines with node
-                              windows, run and the residual): as 5, with a
+                              windows; run, run_x and the residual): as 5, with a
                               window of node entries (three x values at
                               columns 3n .. 3n+2, or one column thrice for
-                              pads) and one 10-bit entry index per three
-                              steps, 27-step batches; otherwise as 5 */
+                              pads; run_x: three Kr columns each) and one
+                              10-bit entry index per three steps (unroll
+                              16: 27-step batches, 8: 18-step); otherwise
+                              as 5 */
 } wk_launch;
 
 void wk_launch_default(wk_launch *cfg);

@@ -8,7 +8,9 @@ per three steps.  Structure: every slot, decoded through the node window,
 names its padded column (pads read column 0).  Products: the node kernel sums
 the same slots in the same order as the packed-window kernel (stage_lds 5),
 so y is bit-identical to it in every precision and, in FP64, to the oracle.
-Layouts without node windows, run_x and k-tiles above 1 run as stage 5."""
+run_x reads entries of three Kr columns each (a node's columns are not
+adjacent in that numbering).  Layouts without node windows and k-tiles above
+1 run as stage 5."""
 import sys
 
 import numpy as np
@@ -83,7 +85,13 @@ def decode_node_columns(eng, L):
     ent = (word >> (10 * (trip % 3))) & 1023
     assert np.all(ent < cnt[group])
     e = nc[off[group] + ent]
-    return np.where(e >= 0, e + step % 3, -e - 1), cnt
+    cols = np.where(e >= 0, e + step % 3, -e - 1)
+    kr = eng.download(capi.WK_ENGINE_ND_COLS_KR)
+    cols_kr = None
+    if kr.size:
+        assert kr.size == 3 * off[-1]
+        cols_kr = kr[3 * (off[group] + ent) + step % 3]
+    return cols, cols_kr, cnt
 
 
 @pytest.mark.parametrize("variant", [1, 2])
@@ -91,10 +99,12 @@ def test_node_windows_name_each_slots_column(variant):
     say(f"[gpu] node windows decode, variant {variant}")
     m = matrix("hex_elastic_3d_32")
     s, L = layout(m, variant)
-    eng = engine(s, m.vals, m.colind)
-    cols, cnt = decode_node_columns(eng, L)
+    eng = engine(s, m.vals, m.colind, kr=True)
+    cols, cols_kr, cnt = decode_node_columns(eng, L)
     _, pc = O.process(L, np.zeros(m.nnz), m.colind)  # pads read column 0
     assert np.array_equal(cols, pc)
+    # Kr numbering: pads read column 0 of that numbering
+    assert np.array_equal(cols_kr, O.reorder_columns_coalesced(L, m.colind))
     # one entry per node a group touches (a third of its window columns), and
     # one for pads where the group has them
     wsz = np.diff(eng.download(capi.WK_ENGINE_WIN_OFF).astype(np.int64))
@@ -124,9 +134,10 @@ def test_node_kernel_bitexact(name, variant):
     say(f"[gpu] node kernel {name} variant {variant}")
     m = matrix(name)
     s, L = layout(m, variant)
-    eng = engine(s, m.vals, m.colind)
+    eng = engine(s, m.vals, m.colind, kr=True)
     assert eng.download(capi.WK_ENGINE_ND_CNT).size == L.nwarps
     x = fe.uniform_x(m.ncols, seed=7102)
+    dxr = dev(s.reorder_x(x))
     if variant == 1:
         ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
     else:
@@ -145,6 +156,13 @@ def test_node_kernel_bitexact(name, variant):
         eng.residual(dev(b), dev(x), r, cfg=cfg)
         eng.residual(dev(b), dev(x), r5, cfg=dict(cfg, stage_lds=5))
         assert torch.equal(r, r5) and np.array_equal(host(r), b - ref)
+        # run_x: entries of three Kr columns
+        yx, yx5 = nan_like(m.nrows), nan_like(m.nrows)
+        eng.launch_x(dxr, yx, cfg=cfg)
+        eng.launch_x(dxr, yx5, cfg=dict(cfg, stage_lds=5))
+        assert torch.equal(yx, yx5)
+        if variant == 1:
+            assert np.array_equal(host(yx), ref[s.row_map])
     eng.close()
 
 
@@ -182,22 +200,26 @@ def test_node_kernel_fp32_matches_packed(acc):
 
 
 def test_node_windows_from_the_device_scan_and_run_x():
-    """Engines scanned on the device get the same node windows; run_x (Kr
-    numbering) runs as stage 5."""
+    """Engines scanned on the device get the same node windows, in both
+    numberings."""
     say("[gpu] node windows, device scan and run_x")
     m = matrix("hex_elastic_3d_32")
     s, L = layout(m, 1)
     he = engine(s, m.vals, m.colind, kr=True)
     de = engine.from_device(dev(m.rowptr), dev(m.colind), dev(m.vals), ncols=m.ncols, kr=True)
-    for w in (capi.WK_ENGINE_ND_COLS, capi.WK_ENGINE_ND_CNT, capi.WK_ENGINE_ND_OFF, capi.WK_ENGINE_ND_POS):
+    off = he.download(capi.WK_ENGINE_WIN_OFF).astype(np.int64)
+    cnt = he.download(capi.WK_ENGINE_ND_CNT).astype(np.int64)
+    keep = np.zeros(off[-1], bool)  # only each group's first ND_CNT entries are written
+    for g in range(L.nwarps):
+        keep[off[g]:off[g] + cnt[g]] = True
+    for w in (capi.WK_ENGINE_ND_COLS, capi.WK_ENGINE_ND_COLS_KR, capi.WK_ENGINE_ND_CNT,
+              capi.WK_ENGINE_ND_OFF, capi.WK_ENGINE_ND_POS):
         a, b = he.download(w), de.download(w)
-        if w == capi.WK_ENGINE_ND_COLS:  # only each group's first ND_CNT entries are written
-            off = he.download(capi.WK_ENGINE_WIN_OFF).astype(np.int64)
-            cnt = he.download(capi.WK_ENGINE_ND_CNT).astype(np.int64)
-            keep = np.zeros(a.size, bool)
-            for g in range(L.nwarps):
-                keep[off[g]:off[g] + cnt[g]] = True
+        assert a.size == b.size > 0, w
+        if w == capi.WK_ENGINE_ND_COLS:
             a, b = a[keep], b[keep]
+        if w == capi.WK_ENGINE_ND_COLS_KR:
+            a, b = a.reshape(-1, 3)[keep], b.reshape(-1, 3)[keep]
         assert np.array_equal(a, b), w
     x = fe.uniform_x(m.ncols, seed=7106)
     ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)

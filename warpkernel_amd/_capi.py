@@ -22,6 +22,7 @@ WK_ENGINE_KR, WK_ENGINE_ADJOINT, WK_ENGINE_ROWSORT = 1, 2, 4
 WK_ENGINE_VALUES, WK_ENGINE_COLS, WK_ENGINE_COLS_KR = 16, 17, 18
 WK_ENGINE_WIN_IDX, WK_ENGINE_WIN_OFF, WK_ENGINE_WIN_COLS, WK_ENGINE_WIN_COLS_KR = 19, 20, 21, 22
 WK_ENGINE_ND_COLS, WK_ENGINE_ND_CNT, WK_ENGINE_ND_OFF, WK_ENGINE_ND_POS = 23, 24, 25, 26
+WK_ENGINE_ND_COLS_KR = 27
 
 
 class WKError(RuntimeError):

@@ -267,7 +267,8 @@ class engine:
               capi.WK_ENGINE_WIN_IDX: np.uint16, capi.WK_ENGINE_WIN_OFF: np.uint32,
               capi.WK_ENGINE_WIN_COLS: np.int32, capi.WK_ENGINE_WIN_COLS_KR: np.int32,
               capi.WK_ENGINE_ND_COLS: np.int32, capi.WK_ENGINE_ND_CNT: np.uint32,
-              capi.WK_ENGINE_ND_OFF: np.uint32, capi.WK_ENGINE_ND_POS: np.uint32}[which]
+              capi.WK_ENGINE_ND_OFF: np.uint32, capi.WK_ENGINE_ND_POS: np.uint32,
+              capi.WK_ENGINE_ND_COLS_KR: np.int32}[which]
         n = C.c_int64()
         check(lib().wk_engine_download(self._h, which, None, C.byref(n)))
         out = np.empty(n.value, dt)

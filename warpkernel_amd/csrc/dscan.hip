@@ -587,6 +587,7 @@ extern "C" int wk_engine_download(const wk_engine *E, int32_t which, void *host_
       }
       case WK_ENGINE_ND_COLS: src = D.nd_cols; n = D.nd_cols ? D.win_total : 0; break;
       case WK_ENGINE_ND_CNT: src = D.nd_cnt; n = D.nd_cnt ? D.ngroups : 0; break;
+      case WK_ENGINE_ND_COLS_KR: src = D.nd_cols_kr; n = D.nd_cols_kr ? 3 * D.win_total : 0; break;
       case WK_ENGINE_ND_OFF: src = D.nd_off; n = D.nd_off ? D.ngroups + 1 : 0; break;
       case WK_ENGINE_ND_POS: {
         src = D.nd_pos;

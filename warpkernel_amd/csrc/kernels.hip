@@ -243,6 +243,7 @@ struct WinArgs {
   const uint32_t *pk;      // packed positions (spmv_winpk_kernel)
   const uint32_t *pk_off;  // [ngroups+1] first 12-step block of each group
   const int32_t *nd_cols;  // node entries from off[g] (spmv_winnd_kernel)
+  const int32_t *nd_cols_kr;  // the same, three Kr columns each, from 3 * off[g]
   const uint32_t *nd_cnt;  // [ngroups]
   const uint32_t *nd_pos;  // node positions, word-major
   const uint32_t *nd_off;  // [ngroups+1] first word of each group
@@ -428,9 +429,10 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_winnd_kernel(SpmvArgs a, Win
   const V *vbase = static_cast<const V *>(a.vals) + m.base;
   const uint32_t *pw = w.nd_pos + (int64_t)w.nd_off[wave] * 64 + lane;
   char *xs = lds + wib * w.wave_bytes;
-  const int32_t *nc = w.nd_cols + w.off[wave];
+  constexpr bool KR = OUT == OUT_RUNX;  // entries as three Kr columns each
+  const int32_t *nc = KR ? w.nd_cols_kr + 3 * (int64_t)w.off[wave] : w.nd_cols + w.off[wave];
   const uint32_t nn = w.nd_cnt[wave];
-  NdFill<X> f;
+  NdFill<X, KR> f;
   f.cols(nc, nn, lane);
   WinBatchNd<P, NW> bt;
   const bool early = m.len >= (uint32_t)U;
@@ -1065,11 +1067,12 @@ void launch_winnd(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, si
   hipLaunchKernelGGL((spmv_winnd_kernel<P, SPLIT, OUT, NW>), grid, block, lds, s, a, w, nwork, remap);
 }
 
-// node windows read x in the product's own numbering: run and the residual;
-// nw: words (9 steps each) per batch, 2 or 3
+// run and the residual read entries of three consecutive columns, run_x
+// entries of three Kr columns; nw: words (9 steps each) per batch, 2 or 3
 template <class P, bool SPLIT, int NW>
 WinPkFn pick_winnd_out(int out) {
   if (out == OUT_RUN) return &launch_winnd<P, SPLIT, OUT_RUN, NW>;
+  if (out == OUT_RUNX) return &launch_winnd<P, SPLIT, OUT_RUNX, NW>;
   if constexpr (std::is_same_v<P, PF64>)
     if (out == OUT_RESID) return &launch_winnd<P, SPLIT, OUT_RESID, NW>;
   return nullptr;
@@ -1224,7 +1227,8 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const bool windowed = cfg.stage_lds >= 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
                           cfg.nontemporal &&
                           (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
-    if (windowed && cfg.stage_lds == 7 && kt == 1 && D.nd_max > 0) {
+    if (windowed && cfg.stage_lds == 7 && kt == 1 && D.nd_max > 0 &&
+        (out != OUT_RUNX || D.nd_cols_kr)) {
       // node windows: per wave the window of node entries, then a slice of
       // values (unroll 16: 27-step batches, 8: 18-step)
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
@@ -1239,6 +1243,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         w.wave_bytes = (int32_t)wave_bytes;
         w.win_bytes = (int32_t)win_bytes;
         w.nd_cols = D.nd_cols;
+        w.nd_cols_kr = D.nd_cols_kr;
         w.nd_cnt = D.nd_cnt;
         w.nd_pos = D.nd_pos;
         w.nd_off = D.nd_off;
@@ -1472,8 +1477,9 @@ constexpr int kNdWaves = 4;
 __global__ __launch_bounds__(kNdWaves * 64) void node_window_kernel(
     const uint16_t *__restrict__ idx, const int64_t *__restrict__ warp_offsets,
     const uint32_t *__restrict__ max_nz_rows, const uint32_t *__restrict__ win_off,
-    const int32_t *__restrict__ win_cols, int32_t ngroups, uint32_t *__restrict__ cnt,
-    const uint32_t *__restrict__ nd_off, int32_t *__restrict__ nd_cols,
+    const int32_t *__restrict__ win_cols, const int32_t *__restrict__ win_cols_kr,
+    int32_t ngroups, uint32_t *__restrict__ cnt, const uint32_t *__restrict__ nd_off,
+    int32_t *__restrict__ nd_cols, int32_t *__restrict__ nd_cols_kr,
     uint32_t *__restrict__ nd_pos) {
   // per window position: bit 0 a node starts here, bit 1 read as a constant triple
   __shared__ uint8_t kind[kNdWaves][kPackedWindow];
@@ -1521,11 +1527,15 @@ __global__ __launch_bounds__(kNdWaves * 64) void node_window_kernel(
     if (k & 1u) {
       entry_of[wib][p][0] = (uint16_t)r;
       if (nd_pos) nd_cols[w0 + r] = wc[p];
+      if (nd_pos && nd_cols_kr)
+        for (uint32_t d = 0; d < 3; d++) nd_cols_kr[3 * ((int64_t)w0 + r) + d] = win_cols_kr[w0 + p + d];
     }
     if (k & 2u) {
       const uint32_t rc = r + (k & 1u);
       entry_of[wib][p][1] = (uint16_t)rc;
       if (nd_pos) nd_cols[w0 + rc] = -wc[p] - 1;
+      if (nd_pos && nd_cols_kr)
+        for (uint32_t d = 0; d < 3; d++) nd_cols_kr[3 * ((int64_t)w0 + rc) + d] = win_cols_kr[w0 + p];
     }
     total += (uint32_t)__popcll(ms) + (uint32_t)__popcll(mc);
   }
@@ -1556,6 +1566,7 @@ __global__ __launch_bounds__(kNdWaves * 64) void node_window_kernel(
 
 static void build_node_windows(wk_engine *E, DevLayout &D, hipStream_t stream) {
   D.nd_cols = nullptr;
+  D.nd_cols_kr = nullptr;
   D.nd_cnt = nullptr;
   D.nd_pos = nullptr;
   D.nd_off = nullptr;
@@ -1567,8 +1578,8 @@ static void build_node_windows(wk_engine *E, DevLayout &D, hipStream_t stream) {
   const int64_t blocks = ((int64_t)G + kNdWaves - 1) / kNdWaves;
   uint32_t *cnt = dev_alloc<uint32_t>(E, G);
   hipLaunchKernelGGL(node_window_kernel, dim3((uint32_t)blocks), dim3(kNdWaves * 64), 0, stream,
-                     D.win_idx, D.warp_offsets, D.max_nz_rows, D.win_off, D.win_cols, G, cnt,
-                     nullptr, nullptr, nullptr);
+                     D.win_idx, D.warp_offsets, D.max_nz_rows, D.win_off, D.win_cols, nullptr, G,
+                     cnt, nullptr, nullptr, nullptr, nullptr);
   hip_check(hipGetLastError(), "node window launch");
   std::vector<uint32_t> h((size_t)G);
   hip_check(hipMemcpyAsync(h.data(), cnt, sizeof(uint32_t) * (size_t)G, hipMemcpyDeviceToHost, stream),
@@ -1586,9 +1597,10 @@ static void build_node_windows(wk_engine *E, DevLayout &D, hipStream_t stream) {
   D.nd_off = dev_upload(E, off.data(), (int64_t)off.size(), stream);
   D.nd_pos = dev_alloc<uint32_t>(E, (int64_t)off.back() * 64);
   D.nd_cols = dev_alloc<int32_t>(E, D.win_total);
+  if (D.win_cols_kr) D.nd_cols_kr = dev_alloc<int32_t>(E, 3 * D.win_total);
   hipLaunchKernelGGL(node_window_kernel, dim3((uint32_t)blocks), dim3(kNdWaves * 64), 0, stream,
-                     D.win_idx, D.warp_offsets, D.max_nz_rows, D.win_off, D.win_cols, G, cnt,
-                     D.nd_off, D.nd_cols, D.nd_pos);
+                     D.win_idx, D.warp_offsets, D.max_nz_rows, D.win_off, D.win_cols,
+                     D.win_cols_kr, G, cnt, D.nd_off, D.nd_cols, D.nd_cols_kr, D.nd_pos);
   hip_check(hipGetLastError(), "node window launch");
   hip_check(hipStreamSynchronize(stream), "node windows");
   D.nd_cnt = cnt;

@@ -683,30 +683,39 @@ __device__ __forceinline__ void nd_batch(typename P::Acc &acc, const typename P:
   for (int u = 0; u < U; u++) P::step(acc, lv[u * 64 + lane], xv[u]);
 }
 
-// Fills a node window: entry j < n (the group's entries nc[0, n)) gets its
-// three x values.  Four entries per lane in flight, twelve loads.
-template <class X>
+// Fills a node window: entry j < n (the group's entries) gets its three x
+// values.  TRIPLE false: nc[j] = e >= 0 for columns e, e+1, e+2, e < 0 for
+// column -e-1 three times; TRIPLE true: the three columns explicitly, at
+// nc[3j .. 3j+2] (Kr numbering).  Four entries per lane in flight, twelve
+// loads.
+template <class X, bool TRIPLE>
 struct NdFill {
-  int32_t e[4];
-  __device__ __forceinline__ void cols(const int32_t *__restrict__ nc, uint32_t n, int32_t lane) {
+  static constexpr int C = TRIPLE ? 3 : 1;
+  int32_t e[4][C];
+  __device__ __forceinline__ void load(const int32_t *__restrict__ nc, uint32_t n, uint32_t j0,
+                                       int32_t lane) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const uint32_t j = (uint32_t)lane + q * 64;
-      e[q] = j < n ? nc[j] : -1;
+      const uint32_t j = j0 + (uint32_t)lane + q * 64;
+#pragma unroll
+      for (int d = 0; d < C; d++) e[q][d] = j < n ? nc[(int64_t)C * j + d] : (TRIPLE ? 0 : -1);
     }
+  }
+  __device__ __forceinline__ void cols(const int32_t *__restrict__ nc, uint32_t n, int32_t lane) {
+    load(nc, n, 0, lane);
+  }
+  __device__ static __forceinline__ int64_t column(const int32_t (&c)[C], int d) {
+    if constexpr (TRIPLE) return c[d];
+    return c[0] >= 0 ? (int64_t)c[0] + d : -(int64_t)c[0] - 1;
   }
   __device__ __forceinline__ void gather(char *xs, const int32_t *__restrict__ nc, uint32_t n,
                                          const X *__restrict__ xb, int64_t ldx, int32_t lane) {
     for (uint32_t j0 = 0;;) {
       X v[4][3];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        // e >= 0: columns e, e+1, e+2; e < 0: column -e-1 three times
-        const int64_t c = e[q] >= 0 ? e[q] : -(int64_t)e[q] - 1;
-        const int64_t s = e[q] >= 0 ? 1 : 0;
+      for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) v[q][d] = xb[(c + d * s) * ldx];
-      }
+        for (int d = 0; d < 3; d++) v[q][d] = xb[column(e[q], d) * ldx];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const uint32_t j = j0 + (uint32_t)lane + q * 64;
@@ -718,11 +727,7 @@ struct NdFill {
       }
       j0 += 256;
       if (j0 >= n) break;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint32_t j = j0 + (uint32_t)lane + q * 64;
-        e[q] = j < n ? nc[j] : -1;
-      }
+      load(nc, n, j0, lane);
     }
   }
 };

@@ -115,6 +115,9 @@ struct DevLayout {
   // holds x[-e-1] three times.  Per lane, one 10-bit entry index per three
   // steps, three to a 32-bit word, three words per 27-step block.
   int32_t *nd_cols = nullptr;      // [win_total] group g's entries from win_off[g]
+  // the same entries in Kr numbering (engines with Kr windows), as three
+  // explicit columns each: entry j of group g at 3 * (win_off[g] + j)
+  int32_t *nd_cols_kr = nullptr;   // [3 * win_total]
   uint32_t *nd_cnt = nullptr;      // [ngroups] entries per group
   uint32_t *nd_pos = nullptr;      // [nd_off[ngroups] * 64] words, word-major, lane fastest
   uint32_t *nd_off = nullptr;      // [ngroups+1] first word of each group
