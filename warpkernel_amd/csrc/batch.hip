@@ -38,11 +38,16 @@ struct BatchDesc {
   int32_t staged;    // stream this matrix's groups through LDS (long rows)
   int32_t windowed;  // and read x from the group's LDS window (stage 3)
   int32_t packed;    // windows with packed positions (stage 5)
+  int32_t nodes;     // node windows, 27-step batches (stage 5, k-tile 1)
   const uint16_t *win_idx;
   const uint32_t *win_off;
   const int32_t *win_cols;
   const uint32_t *win_pk;
   const uint32_t *win_pk_off;
+  const int32_t *nd_cols;
+  const uint32_t *nd_cnt;
+  const uint32_t *nd_pos;
+  const uint32_t *nd_off;
 };
 
 struct BatchItem {
@@ -87,15 +92,18 @@ constexpr int batch_nb() {
 }
 
 // LDS slice per wave behind the window: the largest of the staged batch, the
-// window batch (values and 16-bit positions) and, at STAGE 5, the packed batch.
+// window batch (values and 16-bit positions) and, at STAGE 5, the packed and
+// the node-window (27-step) batches.
 template <class P, int KT, int U, int STAGE>
 constexpr int batch_slice() {
   if constexpr (STAGE == 0 || KT != 1) return 16;
   constexpr int st = staged_bytes<P, U>();
   constexpr int win = U * 64 * (int)(sizeof(typename P::V) + sizeof(uint16_t));
   constexpr int pk = STAGE == 5 ? 12 * batch_nb<P>() * 64 * (int)sizeof(typename P::V) : 0;
+  constexpr int nd = STAGE == 5 ? WinBatchNd<P, 3>::U * 64 * (int)sizeof(typename P::V) : 0;
   constexpr int a = st > win ? st : win;
-  return a > pk ? a : pk;
+  constexpr int b = pk > nd ? pk : nd;
+  return a > b ? a : b;
 }
 
 // Waves per block: one at STAGE 5 (a window and a packed slice per block, as
@@ -161,7 +169,21 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
     }
   }
   if constexpr (STAGE == 5 && KT == 1) {
-    if (d.windowed && d.packed) {
+    if (d.windowed && d.nodes) {
+      // node windows (three x values per entry, one position per three steps)
+      const uint32_t *pw = d.nd_pos + (int64_t)d.nd_off[g] * 64 + lane;
+      const int32_t *nc = d.nd_cols + d.win_off[g];
+      const uint32_t nn = d.nd_cnt[g];
+      NdFill<X, false> f;
+      f.cols(nc, nn, lane);
+      WinBatchNd<P, 3> bt;
+      const bool early = len >= (uint32_t)WinBatchNd<P, 3>::U;
+      if (early) bt.issue(vbase, pw, 0, lane);  // in flight while the window fills
+      f.gather(region, nc, nn, xb, d.ldx, lane);
+      window_group_nd<P, 3>(acc[0], vbase, pw, region, len, bt, early, slice, lane);
+      done = true;
+    }
+    if (!done && d.windowed && d.packed) {
       constexpr int NB = batch_nb<P>();
       char *xs = region;
       const u32x4 *pk = reinterpret_cast<const u32x4 *>(d.win_pk) + (int64_t)d.win_pk_off[g] * 64;
@@ -408,16 +430,24 @@ int wk_batch_bind(wk_batch *B, const void *const *x_devs, double *const *y_devs,
                                   (double)D.allocate_nz >= (pk5 ? 3.0 : 4.5) * (double)D.win_total
                             : B->stage == 5 && (double)D.allocate_nz >= 3.0 * (double)D.win_total);
       d.packed = B->stage == 5 && D.win_pk != nullptr;
+      // node windows at k-tile 1 where the matrix has them (long groups: the
+      // batch runs 27-step batches only)
+      d.nodes = B->stage == 5 && kt == 1 && D.nd_max > 0 && mean_len >= 40.0;
       d.win_idx = D.win_idx;
       d.win_off = D.win_off;
       d.win_cols = D.win_cols;
       d.win_pk = D.win_pk;
       d.win_pk_off = D.win_pk_off;
+      d.nd_cols = D.nd_cols;
+      d.nd_cnt = D.nd_cnt;
+      d.nd_pos = D.nd_pos;
+      d.nd_off = D.nd_off;
       if (d.windowed) {
         // a block's waves, each with this window and its slice, in 64 KB of
         // LDS; a matrix whose window does not fit reads x directly
         const bool f64 = E->dtype == WK_F64;
-        const int64_t win = ((int64_t)D.win_max * kt * (f64 ? 8 : 4) + 15) / 16 * 16;
+        const int64_t entries = d.nodes ? 3 * (int64_t)D.nd_max : (int64_t)D.win_max;
+        const int64_t win = (entries * kt * (f64 ? 8 : 4) + 15) / 16 * 16;
         const int64_t slice =
             kt > 1 ? 16
                    : B->stage == 5 ? (f64 ? batch_slice<PF64, 1, 8, 5>() : batch_slice<PF32C, 1, 8, 5>())
