@@ -442,8 +442,9 @@ typedef struct wk_batch wk_batch;
  * per compute unit, each over a contiguous range of equal cost.  stage_lds: as
  * in wk_launch, for batches with one right-hand side (0 direct loads, 1
  * staged, 2 staged with the next batch in flight, 3 as 2 plus x windows, 5 as
- * 3 with packed window positions where a matrix has them, one wave per
- * block); staging applies to the matrices whose mean group length is at
+ * 3 with packed window positions where a matrix has them, or node windows
+ * (wk_launch stage_lds 7, 27-step batches) where it has those and groups of
+ * at least 40 steps on average, one wave per block); staging applies to the matrices whose mean group length is at
  * least 24 and windows to those with windows whose entries serve at least 4.5
  * slots each on average (3 with packed positions at stage_lds 5); the others
  * use direct loads.  With several
