@@ -231,3 +231,22 @@ def test_node_windows_from_the_device_scan_and_run_x():
     assert np.array_equal(host(yx), ref[s.row_map])
     he.close()
     de.close()
+
+
+def test_adjoint_through_node_windows():
+    """The transposed layout of an elasticity matrix has node windows too: dx =
+    A^T dy at stage 7 equals stage 5 and the oracle's transposed fma chain."""
+    say("[gpu] adjoint through node windows")
+    m = matrix("hex_elastic_3d_32")
+    s, _ = layout(m, 1)
+    eng = engine(s, m.vals, m.colind, adjoint=True)
+    dy = fe.uniform_x(m.nrows, seed=7107)
+    cp, ri, perm = O.transpose(m.nrows, m.ncols, m.rowptr, m.colind)
+    ref = O.csr_spmv(cp, ri, m.vals[perm], dy)
+    assert eng.get_launch(op=capi.WK_OP_ADJOINT).as_dict()["stage_lds"] == 7
+    dx7, dx5 = nan_like(m.ncols), nan_like(m.ncols)
+    eng.run_adjoint(dev(dy), dx7, cfg=ND)
+    eng.run_adjoint(dev(dy), dx5, cfg=PK)
+    assert torch.equal(dx7, dx5)
+    assert np.array_equal(host(dx7), ref)
+    eng.close()
