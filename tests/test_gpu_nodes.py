@@ -55,6 +55,31 @@ def matrix(name):
     return _CACHE[name]
 
 
+class BlockCSR:
+    """A matrix of dense 3x3 blocks over a random node graph: node i couples to
+    itself and to up to max_deg - 1 other random nodes (none for the nodes
+    listed empty, whose three rows are empty), each coupling a dense block, so
+    every row is whole nodes in ascending column order."""
+
+    def __init__(self, nnodes, max_deg, seed, empty=0.0):
+        rng = np.random.default_rng(seed)
+        rowptr, cols = [0], []
+        for i in range(nnodes):
+            if rng.random() < empty:
+                nbrs = np.zeros(0, np.int64)
+            else:
+                nbrs = np.unique(np.concatenate([[i], rng.integers(0, nnodes, rng.integers(0, max_deg))]))
+            rc = (3 * nbrs[:, None] + np.arange(3)[None, :]).ravel()
+            for _ in range(3):
+                cols.append(rc)
+                rowptr.append(rowptr[-1] + rc.size)
+        self.nrows = self.ncols = 3 * nnodes
+        self.rowptr = np.array(rowptr, np.int32)
+        self.colind = np.concatenate(cols).astype(np.int32)
+        self.nnz = int(self.rowptr[-1])
+        self.vals = rng.uniform(-1.0, 1.0, self.nnz)
+
+
 def layout(m, variant, th=27):
     """The engine's scan and the oracle's layout of the same matrix."""
     if variant == 1:
@@ -250,3 +275,61 @@ def test_adjoint_through_node_windows():
     assert torch.equal(dx7, dx5)
     assert np.array_equal(host(dx7), ref)
     eng.close()
+
+
+@pytest.mark.parametrize("max_deg,empty", [(9, 0.0), (9, 0.02), (12, 0.0), (14, 0.02)])
+def test_node_windows_on_random_block_graphs(max_deg, empty):
+    """Rows of mixed length in whole nodes (pads in most groups, some empty
+    rows): up to 27 entries per row the kernel takes each group's values
+    straight into registers, beyond that in 18- and 27-step batches with a
+    partial last batch.  run, run_x and the residual, FP64 against the oracle;
+    FP32 against stage 5."""
+    say(f"[gpu] node windows on a random block graph, max_deg {max_deg}, empty {empty}")
+    m = BlockCSR(20000, max_deg, seed=7110 + max_deg, empty=empty)
+    s, L = layout(m, 1)
+    eng = engine(s, m.vals, m.colind, kr=True)
+    cols, cols_kr, _ = decode_node_columns(eng, L)
+    _, pc = O.process(L, np.zeros(m.nnz), m.colind)
+    assert np.array_equal(cols, pc)
+    assert np.array_equal(cols_kr, O.reorder_columns_coalesced(L, m.colind))
+    x = fe.uniform_x(m.ncols, seed=7111)
+    ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    b = fe.uniform_x(m.nrows, seed=7112)
+    for u, wpb in ((16, 1), (8, 2), (16, 4)):
+        cfg = dict(ND, unroll=u, waves_per_block=wpb)
+        y, yx, r = nan_like(m.nrows), nan_like(m.nrows), nan_like(m.nrows)
+        eng.launch(dev(x), y, cfg=cfg)
+        eng.launch_x(dev(s.reorder_x(x)), yx, cfg=cfg)
+        eng.residual(dev(b), dev(x), r, cfg=cfg)
+        assert np.array_equal(host(y), ref)
+        assert np.array_equal(host(yx), ref[s.row_map])
+        assert np.array_equal(host(r), b - ref)
+    eng.close()
+    e32 = engine(s, m.vals, m.colind, dtype="f32")
+    x32 = dev(x.astype(np.float32))
+    for acc in (0, 1):
+        y5, y7 = nan_like(m.nrows), nan_like(m.nrows)
+        e32.launch(x32, y5, cfg=dict(PK, accumulate=acc))
+        e32.launch(x32, y7, cfg=dict(ND, accumulate=acc))
+        assert torch.equal(y5, y7)
+    e32.close()
+
+
+def test_split_layout_with_long_groups():
+    """WPK2 at threshold 81 on hex elasticity keeps every row whole (81-step
+    groups through the split kernel's 27- and 18-step batches); at threshold
+    27 every group has at most 27 steps (values straight into registers)."""
+    say("[gpu] node windows, WPK2 thresholds 81 and 27")
+    m = matrix("hex_elastic_3d_32")
+    x = fe.uniform_x(m.ncols, seed=7113)
+    for th in (81, 27):
+        s, L = layout(m, 2, th)
+        eng = engine(s, m.vals, m.colind)
+        assert eng.download(capi.WK_ENGINE_ND_CNT).size == L.nwarps
+        pv, pc = O.process(L, m.vals, m.colind)
+        ref = O.wpk_spmv(L, pv, pc, x)
+        for u in (16, 8):
+            y = nan_like(m.nrows)
+            eng.launch(dev(x), y, cfg=dict(ND, unroll=u))
+            assert np.array_equal(host(y), ref)
+        eng.close()

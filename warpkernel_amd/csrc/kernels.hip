@@ -402,7 +402,8 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_winpk_kernel(SpmvArgs a, Win
 // same results as spmv_winpk_kernel, with one 4-byte window entry per node
 // instead of three and 10 bits of position per three slots instead of per
 // slot.  Batches of 9 * NW steps of values through the slice, positions in
-// registers.
+// registers; NW = 0 (layouts whose groups have at most 27 steps): the group's
+// values and positions straight into registers, no slice.
 template <class P, bool SPLIT, int OUT, int NW>
 __global__ __launch_bounds__(kMaxThreads) void spmv_winnd_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
                                                                  int32_t remap) {
@@ -410,7 +411,6 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_winnd_kernel(SpmvArgs a, Win
   using X = typename P::X;
   using Acc = typename P::Acc;
   constexpr int W = 64;
-  constexpr int U = WinBatchNd<P, NW>::U;
   extern __shared__ __attribute__((aligned(16))) char lds[];
   const int32_t lane = threadIdx.x & 63;
   const int32_t wpb = blockDim.x >> 6;
@@ -434,12 +434,35 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_winnd_kernel(SpmvArgs a, Win
   const uint32_t nn = w.nd_cnt[wave];
   NdFill<X, KR> f;
   f.cols(nc, nn, lane);
-  WinBatchNd<P, NW> bt;
-  const bool early = m.len >= (uint32_t)U;
-  if (early) bt.issue(vbase, pw, 0, lane);  // in flight while the window fills
-  f.gather(xs, nc, nn, static_cast<const X *>(a.x), a.ldx, lane);
   Acc acc;
-  window_group_nd<P, NW>(acc, vbase, pw, xs, m.len, bt, early, xs + w.win_bytes, lane);
+  if constexpr (NW == 0) {
+    // one batch of at most 27 steps (m.len <= 27, checked by the launcher),
+    // in flight while the window fills
+    V v[27];
+#pragma unroll
+    for (int u = 0; u < 27; u++)
+      if ((uint32_t)u < m.len) v[u] = __builtin_nontemporal_load(vbase + (int64_t)u * W + lane);
+    uint32_t p[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if ((uint32_t)(9 * q) < m.len) p[q] = pw[q * 64];
+    f.gather(xs, nc, nn, static_cast<const X *>(a.x), a.ldx, lane);
+    P::init(acc);  // every slot by fma onto -0
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+      if ((uint32_t)(3 * t) < m.len) {
+        const X *e = reinterpret_cast<const X *>(xs + WinBatchNd<P, 3>::entry(p, t));
+#pragma unroll
+        for (int d = 0; d < 3; d++) P::step(acc, v[3 * t + d], e[d]);
+      }
+  } else {
+    constexpr int U = WinBatchNd<P, NW>::U;
+    WinBatchNd<P, NW> bt;
+    const bool early = m.len >= (uint32_t)U;
+    if (early) bt.issue(vbase, pw, 0, lane);  // in flight while the window fills
+    f.gather(xs, nc, nn, static_cast<const X *>(a.x), a.ldx, lane);
+    window_group_nd<P, NW>(acc, vbase, pw, xs, m.len, bt, early, xs + w.win_bytes, lane);
+  }
   if constexpr (SPLIT) {
     for (uint32_t o = 1; o < m.tpr; o <<= 1) {
       Acc other = P::shfl_down(acc, (int)o, W);
@@ -1068,7 +1091,8 @@ void launch_winnd(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, si
 }
 
 // run and the residual read entries of three consecutive columns, run_x
-// entries of three Kr columns; nw: words (9 steps each) per batch, 2 or 3
+// entries of three Kr columns; nw: words (9 steps each) per batch, 2 or 3, or
+// 0 (groups of at most 27 steps, no slice)
 template <class P, bool SPLIT, int NW>
 WinPkFn pick_winnd_out(int out) {
   if (out == OUT_RUN) return &launch_winnd<P, SPLIT, OUT_RUN, NW>;
@@ -1085,6 +1109,7 @@ WinPkFn pick_winnd_split(bool split, int out) {
 
 template <class P>
 WinPkFn pick_winnd_nw(bool split, int out, int nw) {
+  if (nw == 0) return pick_winnd_split<P, 0>(split, out);
   if (nw == 2) return pick_winnd_split<P, 2>(split, out);
   if (nw == 3) return pick_winnd_split<P, 3>(split, out);
   return nullptr;
@@ -1230,9 +1255,12 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     if (windowed && cfg.stage_lds == 7 && kt == 1 && D.nd_max > 0 &&
         (out != OUT_RUNX || D.nd_cols_kr)) {
       // node windows: per wave the window of node entries, then a slice of
-      // values (unroll 16: 27-step batches, 8: 18-step)
+      // values (unroll 16: 27-step batches, 8: 18-step); groups of at most 27
+      // steps take their values straight into registers, no slice (WPK2 at
+      // threshold 27, hex elasticity 64^3: 105.8 -> 101.6 us, 32^3: 16.0 ->
+      // 14.5 against 18-step batches)
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
-      const int nw = cfg.unroll == 16 ? 3 : 2;
+      const int nw = D.pk_longest <= 27 ? 0 : cfg.unroll == 16 ? 3 : 2;
       const size_t win_bytes = ((size_t)D.nd_max * 3 * xsz + 15) / 16 * 16;
       const size_t wave_bytes = win_bytes + (size_t)9 * nw * 64 * vbytes;
       const size_t need = wave_bytes * (size_t)wpb;
