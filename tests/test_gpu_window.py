@@ -194,18 +194,19 @@ def test_adjoint_engine_refuses_column_swap():
     eng.close()
 
 
-@pytest.mark.parametrize("k", [2, 3, 4, 5, 8, 16, 32])
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6, 7, 8, 16, 32])
 def test_multi_rhs_all_paths_bitexact(k):
     """k right-hand sides through the default choice, the direct kernel at every
-    k-tile and the windowed kernel at k-tiles 2, 4 and 8: FP64 bit-exact against
-    the oracle (WPK1 CSR order, WPK2 lane order), run and run_x."""
+    k-tile and the windowed kernel at k-tiles 2, 4 and 8 (wider than k: one
+    masked tile): FP64 bit-exact against the oracle (WPK1 CSR order, WPK2 lane
+    order), run and run_x."""
     say(f"[gpu] multi-RHS k={k}")
     m = fe.CONFIGS["hex_elastic_3d_32"]() if k in (4, 8) else fe.tet_poisson_3d(40, drop=0.05)
     x = fe.uniform_x(m.ncols, k, seed=4000 + k).reshape(m.ncols, k)
     cfgs = [None] + [dict(k_tile=t, stage_lds=0) for t in (1, 2, 4, 8, 16, 32) if t <= k] + \
-        [dict(WIN, k_tile=t, unroll=8, waves_per_block=w, stage_lds=st) for t in (2, 4, 8) if t <= k
+        [dict(WIN, k_tile=t, unroll=8, waves_per_block=w, stage_lds=st) for t in (2, 4, 8) if t < 2 * k
          for w in (1, 2, 4) for st in (3, 4)] + \
-        [dict(WIN, k_tile=t, unroll=16, waves_per_block=w, stage_lds=4) for t in (4, 8) if t <= k
+        [dict(WIN, k_tile=t, unroll=16, waves_per_block=w, stage_lds=4) for t in (4, 8) if t < 2 * k
          for w in (1, 2)] + \
         [dict(WIN, k_tile=8, unroll=8, waves_per_block=w, stage_lds=6) for w in (1, 2, 4)]
     for variant in (1, 2):
@@ -306,4 +307,31 @@ def test_multi_rhs_columns_match_single_rhs_at_full_size(k):
     for j in (0, 3, k // 2, k - 1):
         eng.launch(dev(np.ascontiguousarray(x[:, j])), y1)
         assert np.array_equal(yk[:, j], host(y1)), (j, eng.get_launch(k=k).as_dict())
+    eng.close()
+
+
+@pytest.mark.parametrize("k", [3, 5, 6, 7, 9, 13])
+def test_masked_last_tile_on_elasticity(k):
+    """The default launch at k not a power of two on hex elasticity: one
+    masked tile for a rest of 3 or more columns (x of the missing columns read
+    as 0, their y untouched), node windows for a rest of one.  run and the
+    residual bit-exact against the oracle, and y past column k never written."""
+    say(f"[gpu] masked last k-tile, hex elasticity 32^3, k={k}")
+    m = fe.CONFIGS["hex_elastic_3d_32"]()
+    s = structure(64).scan(m.nnz, m.nrows, m.vals, m.rowptr, m.colind)
+    eng = engine(s, m.vals, m.colind)
+    x = fe.uniform_x(m.ncols, k, seed=4100 + k).reshape(m.ncols, k)
+    ref = O.csr_spmv(m.rowptr, m.colind, m.vals, x)
+    # y and b as column blocks of wider arrays: a write past column k shows
+    ybig = torch.full((m.nrows, k + 3), float("nan"), dtype=torch.float64, device="cuda")
+    eng.launch(dev(x), ybig[:, :k])
+    yh = host(ybig)
+    assert np.array_equal(yh[:, :k], ref)
+    assert np.isnan(yh[:, k:]).all()
+    b = fe.uniform_x(m.nrows, k, seed=4200 + k).reshape(m.nrows, k)
+    rbig = torch.full((m.nrows, k + 3), float("nan"), dtype=torch.float64, device="cuda")
+    eng.residual(dev(b), dev(x), rbig[:, :k])
+    rh = host(rbig)
+    assert np.array_equal(rh[:, :k], b - ref)
+    assert np.isnan(rh[:, k:]).all()
     eng.close()

@@ -275,13 +275,14 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
     const int64_t pos = (int64_t)a.nznrows + (int64_t)(bid - nwork) * blockDim.x + threadIdx.x;
     if (pos >= a.nrows) return;
     const int64_t dst = OUT == OUT_RUNX ? pos : (int64_t)a.row_map[pos];
-    for (int32_t t = 0; t < ktiles * KT; t++)
+    for (int32_t t = 0; t < a.k; t++)
       a.y[dst * a.ldy + t] = OUT == OUT_RESID ? a.b[dst * a.ldb + t] : 0.0;
     return;
   }
   const int32_t wid = work_id(bid, nwork, remap);
   const int32_t gb = wid / ktiles, kt = wid - gb * ktiles;
   const int32_t wib = threadIdx.x >> 6;
+  // a.k columns in all: the last k-tile may hold fewer than KT (masked)
   // the group, and the first of this wave's right-hand sides in the tile
   const int32_t wave = __builtin_amdgcn_readfirstlane(WPW > 1 ? gb : gb * wpb + wib);
   const uint32_t t0 = WPW > 1 ? (uint32_t)wib * KW : 0u;
@@ -295,6 +296,7 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   (void)slice;  // D = 0: no slice
   const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
   const X *xk = static_cast<const X *>(a.x) + j0;
+  const int32_t kv = a.k - j0 < KT ? a.k - j0 : KT;  // columns of this tile in x
   Acc acc[KW];
   // window columns first, then the first value batches, then the x gathers
   WinFill<X, KT, 64 * WPW> f;
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   f.cols(w.cols + w0, w1 - w0, first);
   if constexpr (D == 0) {
     // values and positions straight into registers (no slice)
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first);
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
     if constexpr (WPW > 1) __syncthreads();  // the window is the block's
     window_group_regs<P, U, KT, KW>(acc, vbase, ibase, xs, m.len, t0, lane);
   } else if constexpr (D == 1) {
@@ -310,12 +312,12 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
     WinBatch<P, U> bt;
     const bool early = m.len >= (uint32_t)U;
     if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first);
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
     window_group<P, U, KT>(acc, vbase, ibase, xs, m.len, bt, early, slice, lane);
   } else {
     WinPipe<P, U, D> pp;
     pp.prologue(vbase, ibase, m.len, lane);  // in flight while the window fills
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first);
+    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
     if constexpr (WPW > 1) __syncthreads();  // the window is the block's
     window_group_deep<P, U, KT, KW, D>(acc, vbase, ibase, xs, m.len, pp, slice, t0, lane);
   }
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   const int64_t c0 = (int64_t)j0 + t0;
 #pragma unroll
   for (int t = 0; t < KW; t++) {
+    if (c0 + t >= a.k) break;  // masked columns of a last k-tile
     const double res = P::result(acc[t]);
     if constexpr (OUT == OUT_RESID)
       a.y[dst * a.ldy + c0 + t] = a.b[dst * a.ldb + c0 + t] - res;
@@ -1157,9 +1160,9 @@ size_t staged_lds_per_wave(int prec, int u) {
   return (size_t)u * 64 * ((prec == 0 ? sizeof(double) : sizeof(float)) + sizeof(int32_t));
 }
 
-int floor_pow2(int v) {
+int ceil_pow2(int v) {
   int p = 1;
-  while (p * 2 <= v) p *= 2;
+  while (p < v) p *= 2;
   return p;
 }
 
@@ -1235,13 +1238,27 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       j0 = ntiles * ktr;
     }
   }
-  // k = sum of power-of-two tiles: as many k_tile tiles as fit, then the rest
-  int kt = std::min(cfg.k_tile, floor_pow2(k - j0 > 0 ? k - j0 : 1));
-  if (D.W == 32) kt = 1;
+  // k = sum of power-of-two tiles: as many k_tile tiles as fit, then the rest.
+  // A rest of 3 or more columns below the tile width goes, where the windowed
+  // kernel takes it, as one masked tile of the next power of two (x values
+  // of the missing columns read as 0, their y not written): one pass over the
+  // values instead of one per power of two (hex elasticity 64^3: k = 7 as
+  // 4 + 2 + 1 took 385 us, k = 8 takes 207)
+  int kt = D.W == 32 ? 1 : cfg.k_tile;
+  bool mask_ok = true;
   while (j0 < k) {
     const int32_t rem = k - j0;
-    while (kt > rem) kt >>= 1;
-    const int32_t ntiles = rem / kt;
+    int32_t kv = 0;  // columns of a masked tile; 0: whole tiles
+    if (kt > rem) {
+      const int p2 = ceil_pow2(rem);
+      if (mask_ok && rem >= 3 && p2 > rem && p2 <= kt) {
+        kt = p2;
+        kv = rem;
+      } else {
+        while (kt > rem) kt >>= 1;
+      }
+    }
+    const int32_t ntiles = kv ? 1 : rem / kt;
     // rows per lane applies to single right-hand-side tiles of W = 64 layouts
     const int r = (kt == 1 && D.W == 64) ? rpl : 1;
     // LDS staging where it is offered (one right-hand side, W = 64, one group
@@ -1252,7 +1269,10 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     const bool windowed = cfg.stage_lds >= 3 && D.win_max > 0 && D.W == 64 && r == 1 &&
                           cfg.nontemporal &&
                           (kt == 1 ? (cfg.unroll == 8 || cfg.unroll == 16) : kt <= 8);
-    if (windowed && cfg.stage_lds == 7 && kt == 1 && D.nd_max > 0 &&
+    // a k-tile-1 rest of a register-fed launch (stage 4, e.g. k = 9 as tiles
+    // of 8 and 1) takes node windows where the layout has them
+    const int st1 = kt == 1 && k > 1 && cfg.stage_lds == 4 && D.nd_max > 0 ? 7 : cfg.stage_lds;
+    if (windowed && st1 == 7 && kt == 1 && D.nd_max > 0 &&
         (out != OUT_RUNX || D.nd_cols_kr)) {
       // node windows: per wave the window of node entries, then a slice of
       // values (unroll 16: 27-step batches, 8: 18-step); groups of at most 27
@@ -1292,7 +1312,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         continue;
       }
     }
-    if (windowed && cfg.stage_lds >= 5 && kt == 1 && D.win_pk) {  // 5, the rest of 6, 7 elsewhere
+    if (windowed && st1 >= 5 && kt == 1 && D.win_pk) {  // 5, the rest of 6, 7 elsewhere
       // packed positions: per wave the window, then a 12-step slice of values
       const size_t vbytes = prec == 0 ? sizeof(double) : sizeof(float);
       const size_t win_bytes = ((size_t)D.win_max * xsz + 15) / 16 * 16;
@@ -1359,15 +1379,19 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
         a.x = static_cast<const char *>(x) + (size_t)j0 * xsz;
         a.y = y + j0;
         a.b = b ? b + j0 : nullptr;
-        a.k = ntiles * kt;
+        a.k = kv ? kv : ntiles * kt;
         if (nwork + nf > 0)
           wf(a, w, dim3((uint32_t)(nwork + nf)), dim3(bw * 64), std::max(need, lds), (int32_t)nwork,
              ntiles, cfg.xcd_remap ? 1 : 0, stream);
         hip_check(hipGetLastError(), "spmv launch");
-        j0 += ntiles * kt;
+        j0 += a.k;
         continue;
       }
       // the window does not fit this block shape: the kernels without windows
+    }
+    if (kv) {  // only the windowed kernel masks: the rest in whole tiles
+      mask_ok = false;
+      continue;
     }
     if (staged) {
       StagedFn sf = pick_staged(prec, split, out, cfg.unroll, cfg.stage_lds >= 2);

@@ -423,14 +423,17 @@ struct WinFill {
       c[q] = j < n ? wcols[j] : 0;
     }
   }
+  // kv: the values of an entry that exist in x (a masked last k-tile holds
+  // fewer than KT columns); the others are written as 0
   __device__ __forceinline__ void gather(char *xs, const int32_t *__restrict__ wcols, uint32_t n,
-                                         const X *__restrict__ xb, int64_t ldx, int32_t first) {
+                                         const X *__restrict__ xb, int64_t ldx, int32_t first,
+                                         int32_t kv = KT) {
     for (uint32_t j0 = 0;;) {
       X v[C][KT];
 #pragma unroll
       for (int q = 0; q < C; q++)
 #pragma unroll
-        for (int t = 0; t < KT; t++) v[q][t] = xb[(int64_t)c[q] * ldx + t];
+        for (int t = 0; t < KT; t++) v[q][t] = t < kv ? xb[(int64_t)c[q] * ldx + t] : X(0);
 #pragma unroll
       for (int q = 0; q < C; q++) {
         const uint32_t j = j0 + (uint32_t)first + q * S;

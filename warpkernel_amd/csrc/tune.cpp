@@ -25,6 +25,12 @@ int pow2_at_most(int v) {
   return p;
 }
 
+int pow2_at_least(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+
 const DevLayout &layout_for(const wk_engine *E, int32_t op) { return op == WK_OP_ADJOINT ? E->adj : E->fwd; }
 
 // Candidate set: the knobs that change speed but not results.
@@ -91,7 +97,7 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
         c.unroll = 8;
         c.rows_per_lane = 1;
         c.stage_lds = 0;
-        for (int kt = 1; kt <= std::min(32, pow2_at_most(k)); kt *= 2) {
+        for (int kt = 1; kt <= std::min(32, pow2_at_least(k)); kt *= 2) {
           c.k_tile = kt;
           out.push_back(c);
           if (D.win_max > 0 && kt >= 2 && kt <= 8) {
@@ -130,6 +136,7 @@ std::vector<wk_launch> candidates(const DevLayout &D, int32_t dtype, const wk_la
 wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
   wk_launch c;
   wk_launch_default(&c);
+  bool masked = false;  // k-tiles may exceed k (windowed multi-RHS)
   const double mean_len =
       D.ngroups ? (double)D.allocate_nz / ((double)D.ngroups * D.W) : 0.0;
   // x windows pay where each window entry serves enough slots to amortise its
@@ -180,6 +187,11 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     c.waves_per_block = 1;
     c.xcd_remap = 1;
   } else if (k >= 2 && windows) {
+    // (k-tiles may pass k: the launcher runs a rest of 3 or more columns as
+    // one masked tile of the next power of two: hex elasticity 64^3 k=3 as
+    // one 4-wide pass, k=7 as one 8-wide pass 207 us against 385 as 4 + 2 +
+    // 1)
+    masked = true;
     // several right-hand sides through windows of k-tile values per column,
     // all k-tiles in one launch, a group's tiles on one XCD (remap) so its
     // values and positions come from HBM once.  Up to k = 16 the values and
@@ -197,13 +209,15 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     c.unroll = 8;
     c.xcd_remap = 1;
     if (dtype == WK_F64) {
-      if (k < 8 || (k < 16 && D.ngroups < 2048 && mean_len >= 40.0)) {
+      if (k <= 4 || (k >= 8 && k < 16 && D.ngroups < 2048 && mean_len >= 40.0)) {
         // (k = 8 on small layouts of long rows: k-tile 4, one wave per window;
         // elasticity 32^3, 1685 groups, 30.8 us against 32.4 with k-tile 8
         // over two waves; 40^3, 3231 groups, 56.6 against 55.4; hex diffusion
-        // 48^3, 27-entry rows, 17.6 against 16.9)
+        // 48^3, 27-entry rows, 17.6 against 16.9.  k = 5 to 7 take the k = 8
+        // shape below as one masked pass: elasticity 32^3 k=7 41.9 us as
+        // 4 + masked 4)
         c.stage_lds = 4;
-        c.k_tile = std::min(4, pow2_at_most(k));
+        c.k_tile = std::min(4, pow2_at_least(k));
         c.waves_per_block = 1;
       } else if (k < 16 || (k == 16 && D.ngroups < 4096) || mean_len < 40.0) {
         // (short rows keep the register-fed form at every k: hex diffusion
@@ -288,7 +302,7 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
       // waves: hex diffusion 48^3 k=4 7.6 us against 9.0, P1 tets k=8 53.5
       // against 66.0; hex elasticity keeps one, 22.5 against 22.7)
       c.stage_lds = 4;
-      c.k_tile = std::min(8, pow2_at_most(k));
+      c.k_tile = std::min(8, pow2_at_least(k));
       c.waves_per_block = k < 16 && mean_len >= 40.0 ? 1 : 2;
       c.unroll = k < 8 ? 8 : 16;
     }
@@ -331,7 +345,8 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
       }
     }
   }
-  c.k_tile = std::min(c.k_tile, pow2_at_most(std::max(1, k)));  // as the launcher uses it
+  // as the launcher uses it (a masked last tile where the windows run it)
+  c.k_tile = std::min(c.k_tile, masked ? pow2_at_least(std::max(1, k)) : pow2_at_most(std::max(1, k)));
   return c;
 }
 
