@@ -246,7 +246,7 @@ def extra_multi_rhs(torch, dev, steps, warmup, tune=True, flusher=None):
     e32 = engine(s, vq.astype(np.float64), m.colind, dtype="f32")
     rows = []
     base = base_cold = None
-    for k in (1, 4, 8, 16, 32):
+    for k in (1, 3, 4, 7, 8, 16, 32):
         x = fe.uniform_x(m.ncols, k, seed=2100 + k).reshape(m.ncols, k)
         xq = x.astype(np.float32)
         dx64, dx32 = dev(x), dev(xq)
