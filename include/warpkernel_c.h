@@ -302,11 +302,13 @@ typedef struct wk_launch {
                               and 16 with stage_lds 4 at k_tiles 4 and 8) */
   int32_t k_tile;          /* right-hand sides per pass: 1, 2, 4, 8, 16 or 32.
                               k is run as whole k-tiles, then the rest in
-                              halving tiles; with x windows (stage_lds 3, 4
-                              or 7 at k-tiles 2 to 8) a rest of 3 or more
-                              columns runs as one masked tile of the next
-                              power of two (its missing columns of x read as
-                              0, of y not written), so k_tile may exceed k */
+                              halving tiles; with register-fed x windows
+                              (stage_lds 4 or 7, unroll 8, k-tiles 4 and 8,
+                              waves_per_block 1 or 2; run and the residual)
+                              a rest of 3 or more columns runs as one masked
+                              tile of the next power of two (its missing
+                              columns of x read as 0, of y not written), so
+                              k_tile may exceed k */
   int32_t xcd_remap;       /* 1: contiguous group ranges per XCD */
   int32_t nontemporal;     /* 1: streaming (nontemporal) loads for the values;
                               2: for the column indices too (direct loads at

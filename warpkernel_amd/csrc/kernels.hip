@@ -258,7 +258,7 @@ struct WinArgs {
 // (WPW = 1), or [window | slice of wave 0 | ... ] (WPW > 1).  D batches of
 // values and positions in flight through the slices, or D = 0: loaded
 // straight into registers, no slices.
-template <class P, bool SPLIT, int OUT, int U, int KT, int D, int WPW>
+template <class P, bool SPLIT, int OUT, int U, int KT, int D, int WPW, bool MASK = false>
 __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinArgs w, int32_t nwork,
                                                                int32_t ktiles, int32_t remap) {
   using V = typename P::V;
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   const int32_t wid = work_id(bid, nwork, remap);
   const int32_t gb = wid / ktiles, kt = wid - gb * ktiles;
   const int32_t wib = threadIdx.x >> 6;
-  // a.k columns in all: the last k-tile may hold fewer than KT (masked)
+  // a.k columns in all; MASK: the last k-tile may hold fewer than KT
   // the group, and the first of this wave's right-hand sides in the tile
   const int32_t wave = __builtin_amdgcn_readfirstlane(WPW > 1 ? gb : gb * wpb + wib);
   const uint32_t t0 = WPW > 1 ? (uint32_t)wib * KW : 0u;
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   (void)slice;  // D = 0: no slice
   const uint32_t w0 = w.off[wave], w1 = w.off[wave + 1];
   const X *xk = static_cast<const X *>(a.x) + j0;
-  const int32_t kv = a.k - j0 < KT ? a.k - j0 : KT;  // columns of this tile in x
+  const int32_t kv = MASK && a.k - j0 < KT ? a.k - j0 : KT;  // columns of this tile in x
   Acc acc[KW];
   // window columns first, then the first value batches, then the x gathers
   WinFill<X, KT, 64 * WPW> f;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   f.cols(w.cols + w0, w1 - w0, first);
   if constexpr (D == 0) {
     // values and positions straight into registers (no slice)
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
+    f.template gather<MASK>(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
     if constexpr (WPW > 1) __syncthreads();  // the window is the block's
     window_group_regs<P, U, KT, KW>(acc, vbase, ibase, xs, m.len, t0, lane);
   } else if constexpr (D == 1) {
@@ -312,12 +312,12 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
     WinBatch<P, U> bt;
     const bool early = m.len >= (uint32_t)U;
     if (early) bt.issue(vbase, ibase, 0, lane);  // in flight while the window fills
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
+    f.template gather<MASK>(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
     window_group<P, U, KT>(acc, vbase, ibase, xs, m.len, bt, early, slice, lane);
   } else {
     WinPipe<P, U, D> pp;
     pp.prologue(vbase, ibase, m.len, lane);  // in flight while the window fills
-    f.gather(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
+    f.template gather<MASK>(xs, w.cols + w0, w1 - w0, xk, a.ldx, first, kv);
     if constexpr (WPW > 1) __syncthreads();  // the window is the block's
     window_group_deep<P, U, KT, KW, D>(acc, vbase, ibase, xs, m.len, pp, slice, t0, lane);
   }
@@ -336,7 +336,8 @@ __global__ __launch_bounds__(kMaxThreads) void spmv_win_kernel(SpmvArgs a, WinAr
   const int64_t c0 = (int64_t)j0 + t0;
 #pragma unroll
   for (int t = 0; t < KW; t++) {
-    if (c0 + t >= a.k) break;  // masked columns of a last k-tile
+    if constexpr (MASK)
+      if (c0 + t >= a.k) break;  // masked columns of a last k-tile
     const double res = P::result(acc[t]);
     if constexpr (OUT == OUT_RESID)
       a.y[dst * a.ldy + c0 + t] = a.b[dst * a.ldb + c0 + t] - res;
@@ -963,11 +964,40 @@ LdsLimits lds_limits() {
 using WinFn = void (*)(const SpmvArgs &, const WinArgs &, dim3, dim3, size_t, int32_t, int32_t,
                        int32_t, hipStream_t);
 
-template <class P, bool SPLIT, int OUT, int U, int KT, int D, int WPW>
+template <class P, bool SPLIT, int OUT, int U, int KT, int D, int WPW, bool MASK = false>
 void launch_win(const SpmvArgs &a, const WinArgs &w, dim3 grid, dim3 block, size_t lds,
                 int32_t nwork, int32_t ktiles, int32_t remap, hipStream_t s) {
-  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U, KT, D, WPW>), grid, block, lds, s, a, w,
-                     nwork, ktiles, remap);
+  hipLaunchKernelGGL((spmv_win_kernel<P, SPLIT, OUT, U, KT, D, WPW, MASK>), grid, block, lds, s, a,
+                     w, nwork, ktiles, remap);
+}
+
+// masked last k-tiles: the register-fed shapes of the layout defaults (unroll
+// 8, k-tiles 4 and 8, one or two waves per window; run and the residual)
+template <class P, bool SPLIT, int OUT>
+WinFn pick_win_masked(int u, int kt, int wpw) {
+  if (u != 8 || (wpw != 1 && wpw != 2)) return nullptr;
+  if (kt == 4)
+    return wpw == 1 ? &launch_win<P, SPLIT, OUT, 8, 4, 0, 1, true> : &launch_win<P, SPLIT, OUT, 8, 4, 0, 2, true>;
+  if (kt == 8)
+    return wpw == 1 ? &launch_win<P, SPLIT, OUT, 8, 8, 0, 1, true> : &launch_win<P, SPLIT, OUT, 8, 8, 0, 2, true>;
+  return nullptr;
+}
+
+template <class P>
+WinFn pick_win_masked_out(bool split, int out, int u, int kt, int wpw) {
+  if (out == OUT_RUN)
+    return split ? pick_win_masked<P, true, OUT_RUN>(u, kt, wpw) : pick_win_masked<P, false, OUT_RUN>(u, kt, wpw);
+  if constexpr (std::is_same_v<P, PF64>)
+    if (out == OUT_RESID)
+      return split ? pick_win_masked<P, true, OUT_RESID>(u, kt, wpw)
+                   : pick_win_masked<P, false, OUT_RESID>(u, kt, wpw);
+  return nullptr;
+}
+
+WinFn pick_win_mask(int prec, bool split, int out, int u, int kt, int wpw) {
+  if (prec == 0) return pick_win_masked_out<PF64>(split, out, u, kt, wpw);
+  if (prec == 1) return pick_win_masked_out<PF32C>(split, out, u, kt, wpw);
+  return pick_win_masked_out<PF32D>(split, out, u, kt, wpw);
 }
 
 // k-tile 1 with unroll 8 or 16; wider k-tiles (2, 4, 8) with unroll 8 and
@@ -1239,11 +1269,12 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
     }
   }
   // k = sum of power-of-two tiles: as many k_tile tiles as fit, then the rest.
-  // A rest of 3 or more columns below the tile width goes, where the windowed
-  // kernel takes it, as one masked tile of the next power of two (x values
-  // of the missing columns read as 0, their y not written): one pass over the
-  // values instead of one per power of two (hex elasticity 64^3: k = 7 as
-  // 4 + 2 + 1 took 385 us, k = 8 takes 207)
+  // A rest of 3 or more columns below the tile width goes, where the
+  // register-fed windowed kernel takes it (unroll 8, k-tiles 4 and 8, one or
+  // two waves per window; run and the residual), as one masked tile of the
+  // next power of two (x values of the missing columns read as 0, their y not
+  // written): one pass over the values instead of one per power of two (hex
+  // elasticity 64^3: k = 7 as 4 + 2 + 1 took 385 us, k = 8 takes 207)
   int kt = D.W == 32 ? 1 : cfg.k_tile;
   bool mask_ok = true;
   while (j0 < k) {
@@ -1363,7 +1394,8 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       const size_t slice_bytes = regs ? 0 : (size_t)u * 64 * (vbytes + sizeof(uint16_t));
       const size_t wave_bytes = win_bytes + slice_bytes;
       const size_t need = wpw > 1 ? win_bytes + slice_bytes * (size_t)wpw : wave_bytes * (size_t)wpb;
-      WinFn wf = pick_win(prec, split, out, u, kt, wpw, regs);
+      WinFn wf = kv ? (regs ? pick_win_mask(prec, split, out, u, kt, wpw) : nullptr)
+                    : pick_win(prec, split, out, u, kt, wpw, regs);
       if (wf && need <= (size_t)lds_limits().per_block) {
         WinArgs w{};
         w.idx = D.win_idx;

@@ -423,8 +423,9 @@ struct WinFill {
       c[q] = j < n ? wcols[j] : 0;
     }
   }
-  // kv: the values of an entry that exist in x (a masked last k-tile holds
-  // fewer than KT columns); the others are written as 0
+  // MASKED: only kv values of an entry exist in x (a masked last k-tile
+  // holds fewer than KT columns); the others are written as 0
+  template <bool MASKED = false>
   __device__ __forceinline__ void gather(char *xs, const int32_t *__restrict__ wcols, uint32_t n,
                                          const X *__restrict__ xb, int64_t ldx, int32_t first,
                                          int32_t kv = KT) {
@@ -433,7 +434,8 @@ struct WinFill {
 #pragma unroll
       for (int q = 0; q < C; q++)
 #pragma unroll
-        for (int t = 0; t < KT; t++) v[q][t] = t < kv ? xb[(int64_t)c[q] * ldx + t] : X(0);
+        for (int t = 0; t < KT; t++)
+          v[q][t] = !MASKED || t < kv ? xb[(int64_t)c[q] * ldx + t] : X(0);
 #pragma unroll
       for (int q = 0; q < C; q++) {
         const uint32_t j = j0 + (uint32_t)first + q * S;
