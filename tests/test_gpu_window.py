@@ -310,11 +310,12 @@ def test_multi_rhs_columns_match_single_rhs_at_full_size(k):
     eng.close()
 
 
-@pytest.mark.parametrize("k", [3, 5, 6, 7, 9, 13])
+@pytest.mark.parametrize("k", [3, 5, 6, 7, 9, 11, 13, 15])
 def test_masked_last_tile_on_elasticity(k):
-    """The default launch at k not a power of two on hex elasticity: one
-    masked tile for a rest of 3 or more columns (x of the missing columns read
-    as 0, their y untouched), node windows for a rest of one.  run and the
+    """The default launch at k not a power of two on hex elasticity: a masked
+    last tile for a rest of 3 or more columns, alone or after whole tiles in
+    one launch (x of the missing columns read as 0, their y untouched), node
+    windows for a rest of one.  run and the
     residual bit-exact against the oracle, and y past column k never written."""
     say(f"[gpu] masked last k-tile, hex elasticity 32^3, k={k}")
     m = fe.CONFIGS["hex_elastic_3d_32"]()

@@ -1279,7 +1279,7 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
   bool mask_ok = true;
   while (j0 < k) {
     const int32_t rem = k - j0;
-    int32_t kv = 0;  // columns of a masked tile; 0: whole tiles
+    int32_t kv = 0;  // columns of a launch whose last tile is masked; 0: whole tiles
     if (kt > rem) {
       const int p2 = ceil_pow2(rem);
       if (mask_ok && rem >= 3 && p2 > rem && p2 <= kt) {
@@ -1288,8 +1288,10 @@ void launch_spmv(const DevLayout &D, int32_t dtype, const wk_launch &cfg, int ou
       } else {
         while (kt > rem) kt >>= 1;
       }
+    } else if (mask_ok && kt >= 4 && rem % kt >= 3) {
+      kv = rem;  // whole tiles and a masked last one in one launch (k = 7 as 4 + 3)
     }
-    const int32_t ntiles = kv ? 1 : rem / kt;
+    const int32_t ntiles = kv ? (rem + kt - 1) / kt : rem / kt;
     // rows per lane applies to single right-hand-side tiles of W = 64 layouts
     const int r = (kt == 1 && D.W == 64) ? rpl : 1;
     // LDS staging where it is offered (one right-hand side, W = 64, one group

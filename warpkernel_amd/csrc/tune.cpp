@@ -209,13 +209,14 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     c.unroll = 8;
     c.xcd_remap = 1;
     if (dtype == WK_F64) {
-      if (k <= 4 || (k >= 8 && k < 16 && D.ngroups < 2048 && mean_len >= 40.0)) {
+      if (k <= 4 || (k < 16 && D.ngroups < 2048 && mean_len >= 40.0)) {
         // (k = 8 on small layouts of long rows: k-tile 4, one wave per window;
         // elasticity 32^3, 1685 groups, 30.8 us against 32.4 with k-tile 8
         // over two waves; 40^3, 3231 groups, 56.6 against 55.4; hex diffusion
-        // 48^3, 27-entry rows, 17.6 against 16.9.  k = 5 to 7 take the k = 8
-        // shape below as one masked pass: elasticity 32^3 k=7 41.9 us as
-        // 4 + masked 4)
+        // 48^3, 27-entry rows, 17.6 against 16.9.  There k = 7 runs as 4 + a
+        // masked 4 in one launch, 33.2 us against 35.3 as one masked 8.
+        // Elsewhere k = 5 to 7 take the k = 8 shape below as one masked
+        // pass: hex diffusion k=7 19.3 us against 30.4 as 4 + 2 + 1)
         c.stage_lds = 4;
         c.k_tile = std::min(4, pow2_at_least(k));
         c.waves_per_block = 1;
