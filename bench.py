@@ -534,6 +534,8 @@ def run_case(args, torch, dev):
         eng = engine(s, m.vals, m.colind)
         x = dev(fe.uniform_x(m.ncols, seed=2002))
         y = torch.empty(m.nrows, dtype=torch.float64, device="cuda")
+        if args.launch:
+            eng.set_launch(json.loads(args.launch))
         fns = [lambda: eng.launch(x, y)]
         info["launch"] = eng.get_launch().as_dict()
     elif name in ("hex32_fp64", "hex32_fp32c"):
@@ -704,7 +706,7 @@ def main():
     ap.add_argument("--gather", action="store_true",
                     help="tet96 / tet96_perm cases: also launch the gather ceiling each step")
     ap.add_argument("--launch", default=None,
-                    help="tet96 / tet96_perm cases: launch fields to override (JSON object)")
+                    help="headline / tet96 / tet96_perm cases: launch fields to override (JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--write-traffic", nargs=4, metavar=("WORKLOAD", "FETCH_DIR", "WRITE_DIR", "CASE_JSON"))
