@@ -180,11 +180,16 @@ wk_launch heuristic_launch(const DevLayout &D, int32_t dtype, int32_t k) {
     // 9.3
     c.stage_lds = D.win_pk ? 5 : 3;
     c.unroll = mean_len >= 40.0 && dtype == WK_F64 ? 16 : 8;
+    c.waves_per_block = 1;
     if (D.nd_max > 0) {
       c.stage_lds = 7;
       c.unroll = mean_len >= 40.0 ? 16 : 8;
+      // groups of at most 27 steps (their values straight into registers,
+      // no slice): four waves per block on large layouts (WPK2 at threshold
+      // 27, hex elasticity 64^3, 49,189 groups: 101.8 us against 103.5 with
+      // one; 48^3, 20,764 groups: 45.3 against 45.9; 32^3, 6,164: level)
+      if (D.pk_longest <= 27 && D.ngroups >= 16384) c.waves_per_block = 4;
     }
-    c.waves_per_block = 1;
     c.xcd_remap = 1;
   } else if (k >= 2 && windows) {
     // (k-tiles may pass k: the launcher runs a rest of 3 or more columns as
