@@ -178,8 +178,11 @@ __device__ __forceinline__ void batch_group(const BatchDesc &d, int32_t g, int32
       f.cols(nc, nn, lane);
       WinBatchNd<P, 3> bt;
       const bool early = len >= (uint32_t)WinBatchNd<P, 3>::U;
+      X xv[4][3];
+      f.issue_x(xv, xb, d.ldx);  // the first x gathers ahead of the first batch
       if (early) bt.issue(vbase, pw, 0, lane);  // in flight while the window fills
-      f.gather(region, nc, nn, xb, d.ldx, lane);
+      f.write_x(region, xv, nn, lane);
+      f.rest(region, nc, nn, xb, d.ldx, lane);
       window_group_nd<P, 3>(acc[0], vbase, pw, region, len, bt, early, slice, lane);
       done = true;
     }

@@ -713,6 +713,48 @@ struct NdFill {
     if constexpr (TRIPLE) return c[d];
     return c[0] >= 0 ? (int64_t)c[0] + d : -(int64_t)c[0] - 1;
   }
+  // The first round of gather in two halves, so that other loads can go out
+  // between them: issue_x issues the x loads of the lane's first four entries
+  // (cols loaded), write_x stores them into the window.  A window of more than
+  // 256 entries continues with rest.
+  __device__ __forceinline__ void issue_x(X (&v)[4][3], const X *__restrict__ xb, int64_t ldx) const {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) v[q][d] = xb[column(e[q], d) * ldx];
+  }
+  __device__ __forceinline__ void write_x(char *xs, const X (&v)[4][3], uint32_t n, int32_t lane) const {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t j = (uint32_t)lane + q * 64;
+      if (j < n) {
+        X *dst = reinterpret_cast<X *>(xs + j * 3u * (uint32_t)sizeof(X));
+#pragma unroll
+        for (int d = 0; d < 3; d++) dst[d] = v[q][d];
+      }
+    }
+  }
+  __device__ __forceinline__ void rest(char *xs, const int32_t *__restrict__ nc, uint32_t n,
+                                       const X *__restrict__ xb, int64_t ldx, int32_t lane) {
+    if (n <= 256) return;
+    load(nc, n, 256, lane);
+    for (uint32_t j0 = 256;;) {
+      X v[4][3];
+      issue_x(v, xb, ldx);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t j = j0 + (uint32_t)lane + q * 64;
+        if (j < n) {
+          X *dst = reinterpret_cast<X *>(xs + j * 3u * (uint32_t)sizeof(X));
+#pragma unroll
+          for (int d = 0; d < 3; d++) dst[d] = v[q][d];
+        }
+      }
+      j0 += 256;
+      if (j0 >= n) break;
+      load(nc, n, j0, lane);
+    }
+  }
   __device__ __forceinline__ void gather(char *xs, const int32_t *__restrict__ nc, uint32_t n,
                                          const X *__restrict__ xb, int64_t ldx, int32_t lane) {
     for (uint32_t j0 = 0;;) {
